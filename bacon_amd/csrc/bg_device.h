@@ -1,0 +1,245 @@
+// bg_device.h — device-side building blocks of the batched IVP engine:
+// Butcher tableaux as compile-time data, registered right-hand sides, and the
+// RK step attempt (bacon-sci src/ivp/rk.rs:361-423) for one lane.
+//
+// Operation order is fixed: every multiply-add is an explicit fma(), zero
+// tableau entries are skipped at compile time, x^(1/4) is sqrt(sqrt(x)).
+// Add, multiply, divide, sqrt and fma are correctly rounded on gfx950, so a
+// lane's arithmetic is reproducible bit for bit by a CPU that performs the
+// same operations in the same order.  The file is compiled with
+// -ffp-contract=off so the compiler adds no fusions of its own.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/bacon_gpu.h"
+
+namespace bg {
+
+// ---------------------------------------------------------------------------
+// Tableaux.  L is the reference's literal "Row i" data (rk.rs:459-502,
+// 580-601).  The reference stores it with Matrix::from_vec, which fills
+// column-major, and iterates row_iter, so its stage loop reads L transposed;
+// Tableau<_, true> reproduces that, Tableau<_, false> reads rows as rows.
+// ---------------------------------------------------------------------------
+struct Fehlberg45 {
+    static constexpr int S = 6;
+    static constexpr double c[6] = {0.0, 1.0 / 4.0, 3.0 / 8.0, 12.0 / 13.0, 1.0, 1.0 / 2.0};
+    static constexpr double L[6][6] = {
+        {0, 0, 0, 0, 0, 0},
+        {1.0 / 4.0, 0, 0, 0, 0, 0},
+        {3.0 / 32.0, 9.0 / 32.0, 0, 0, 0, 0},
+        {1932.0 / 2197.0, -7200.0 / 2197.0, 7296.0 / 2197.0, 0, 0, 0},
+        {439.0 / 216.0, -8.0, 3680.0 / 513.0, -845.0 / 4104.0, 0, 0},
+        {-8.0 / 27.0, 2.0, -3544.0 / 2565.0, 1859.0 / 4104.0, -11.0 / 40.0, 0}};
+    // 4th-order propagation weights (rk.rs:505-514)
+    static constexpr double b[6] = {25.0 / 216.0, 0.0, 1408.0 / 2565.0, 2197.0 / 4104.0,
+                                    -(1.0 / 5.0), 0.0};
+    // error weights b5 - b4 (rk.rs:516-525)
+    static constexpr double e[6] = {1.0 / 360.0, 0.0, -128.0 / 4275.0, -2197.0 / 75240.0,
+                                    1.0 / 50.0, 2.0 / 55.0};
+};
+
+struct BogackiShampine23 {
+    static constexpr int S = 4;
+    static constexpr double c[4] = {0.0, 1.0 / 2.0, 3.0 / 4.0, 1.0};
+    static constexpr double L[4][4] = {{0, 0, 0, 0},
+                                       {1.0 / 2.0, 0, 0, 0},
+                                       {0, 3.0 / 4.0, 0, 0},
+                                       {2.0 / 9.0, 1.0 / 3.0, 4.0 / 9.0, 0}};
+    static constexpr double b[4] = {2.0 / 9.0, 1.0 / 3.0, 4.0 / 9.0, 0.0};
+    static constexpr double e[4] = {-5.0 / 72.0, 1.0 / 12.0, 1.0 / 9.0, -(1.0 / 8.0)};
+};
+
+template <class Lit, bool AsBuilt>
+struct Tableau {
+    static constexpr int S = Lit::S;
+    __host__ __device__ static constexpr double m(int i, int j) {
+        return AsBuilt ? Lit::L[j][i] : Lit::L[i][j];
+    }
+    __host__ __device__ static constexpr double c(int i) { return Lit::c[i]; }
+    __host__ __device__ static constexpr double b(int i) { return Lit::b[i]; }
+    __host__ __device__ static constexpr double e(int i) { return Lit::e[i]; }
+    // index of the first non-zero error weight
+    __host__ __device__ static constexpr int e_first() {
+        int k = 0;
+        while (k < S && Lit::e[k] == 0.0) ++k;
+        return k;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Registered right-hand sides.  eval() returns false for a UserError.
+// P = number of per-instance parameters (loaded from a [P][n] SoA array).
+// ---------------------------------------------------------------------------
+template <int D_>
+struct RhsExp {
+    static constexpr int D = D_, P = 0;
+    __device__ static bool eval(double, const double (&y)[D], const double *, double (&f)[D]) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) f[i] = y[i];
+        return true;
+    }
+};
+
+template <int D_>
+struct RhsNeg {
+    static constexpr int D = D_, P = 0;
+    __device__ static bool eval(double, const double (&y)[D], const double *, double (&f)[D]) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) f[i] = -y[i];
+        return true;
+    }
+};
+
+template <int D_>
+struct RhsQuadratic {
+    static constexpr int D = D_, P = 0;
+    __device__ static bool eval(double t, const double (&)[D], const double *, double (&f)[D]) {
+#pragma unroll
+        for (int i = 0; i < D; ++i) f[i] = -2.0 * t;
+        return true;
+    }
+};
+
+template <int D_>
+struct RhsCos {
+    static constexpr int D = D_, P = 0;
+    __device__ static bool eval(double t, const double (&)[D], const double *, double (&f)[D]) {
+        const double c = cos(t);
+#pragma unroll
+        for (int i = 0; i < D; ++i) f[i] = c;
+        return true;
+    }
+};
+
+template <int D_>
+struct RhsFailAfter {
+    static constexpr int D = D_, P = 1;
+    __device__ static bool eval(double t, const double (&y)[D], const double *p, double (&f)[D]) {
+        if (t >= p[0]) return false;
+#pragma unroll
+        for (int i = 0; i < D; ++i) f[i] = y[i];
+        return true;
+    }
+};
+
+struct RhsLorenz {
+    static constexpr int D = 3, P = 3;
+    // FLOPs per evaluation (fma = 2): sub+mul, sub+fma, mul+fma
+    static constexpr int FLOPS = 8;
+    __device__ static bool eval(double, const double (&y)[3], const double *p, double (&f)[3]) {
+        const double s = p[0], r = p[1], b = p[2];
+        f[0] = s * (y[1] - y[0]);
+        f[1] = fma(y[0], r - y[2], -y[1]);
+        f[2] = fma(y[0], y[1], -(b * y[2]));
+        return true;
+    }
+};
+
+struct RhsLotkaVolterra {
+    static constexpr int D = 2, P = 4;
+    static constexpr int FLOPS = 7;
+    __device__ static bool eval(double, const double (&y)[2], const double *p, double (&f)[2]) {
+        const double a = p[0], b = p[1], g = p[2], d = p[3];
+        const double xy = y[0] * y[1];
+        f[0] = fma(a, y[0], -(b * xy));
+        f[1] = fma(d, xy, -(g * y[1]));
+        return true;
+    }
+};
+
+struct RhsOscillator {
+    static constexpr int D = 2, P = 0;
+    __device__ static bool eval(double, const double (&y)[2], const double *, double (&f)[2]) {
+        f[0] = y[1];
+        f[1] = -y[0];
+        return true;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// One lane's RK state and one call of RungeKuttaSolver::step.
+// ---------------------------------------------------------------------------
+enum : int { STEP_ACCEPTED = -1, STEP_REJECTED = -2 };
+
+template <class RHS, class Tab>
+struct RkLane {
+    static constexpr int D = RHS::D, S = Tab::S;
+    double t, dt, end, tol, dt_min, dt_max;
+    double y[D];
+    double K[S][D];  // half_steps; persists across attempts (read by AS_BUILT)
+    double p[RHS::P > 0 ? RHS::P : 1];
+
+    // rk.rs:361-423 without the Done test (the caller does it).  Returns
+    // STEP_ACCEPTED, STEP_REJECTED or an IVPError code.
+    __device__ __forceinline__ int attempt() {
+        // rk.rs:366-368
+        if (t + dt >= end) dt = end - t;
+        // rk.rs:370-384
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            double Y[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) Y[d] = y[d];
+#pragma unroll
+            for (int j = 0; j < S; ++j) {
+                const double m = Tab::m(i, j);
+                if (m != 0.0) {
+#pragma unroll
+                    for (int d = 0; d < D; ++d) Y[d] = fma(m, K[j][d], Y[d]);
+                }
+            }
+            const double ti = fma(Tab::c(i), dt, t);
+            double f[D];
+            if (!RHS::eval(ti, Y, p, f)) return BG_USER_ERROR;
+#pragma unroll
+            for (int d = 0; d < D; ++d) K[i][d] = f[d] * dt;
+        }
+        // rk.rs:386-390
+        double E[D];
+        constexpr int e0 = Tab::e_first();
+#pragma unroll
+        for (int d = 0; d < D; ++d) E[d] = Tab::e(e0) * K[e0][d];
+#pragma unroll
+        for (int j = e0 + 1; j < S; ++j) {
+            const double e = Tab::e(j);
+            if (e != 0.0) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) E[d] = fma(e, K[j][d], E[d]);
+            }
+        }
+        double ss = E[0] * E[0];
+#pragma unroll
+        for (int d = 1; d < D; ++d) ss = fma(E[d], E[d], ss);
+        const double error = sqrt(ss) / dt;
+        // rk.rs:392-398
+        const bool accepted = error <= tol;
+        if (accepted) {
+            t += dt;
+#pragma unroll
+            for (int j = 0; j < S; ++j) {
+                const double b = Tab::b(j);
+                if (b != 0.0) {
+#pragma unroll
+                    for (int d = 0; d < D; ++d) y[d] = fma(b, K[j][d], y[d]);
+                }
+            }
+        }
+        // rk.rs:400-412 (safety factor 100/100 = 1.0)
+        const double delta = sqrt(sqrt(tol / error));
+        if (delta <= 0.1)
+            dt *= 0.1;
+        else if (delta >= 4.0)
+            dt *= 4.0;
+        else
+            dt *= delta;
+        if (dt > dt_max) dt = dt_max;
+        // rk.rs:414-416
+        if (dt < dt_min && t < end) return BG_MINIMUM_TIME_DELTA_EXCEEDED;
+        return accepted ? STEP_ACCEPTED : STEP_REJECTED;
+    }
+};
+
+}  // namespace bg

@@ -1,0 +1,45 @@
+// bg_internal.h — host-side interfaces shared by the translation units of
+// libbacon_gpu.so (not part of the public C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/bacon_gpu.h"
+
+namespace bg {
+
+// Launches of the RK family.  Each returns a BG_* status.
+int rk_launch_ensemble(int method, int rhs, int semantics, const bg_ensemble_desc &d,
+                       const bg_launch &launch, hipStream_t stream);
+int rk_launch_plain(int method, int rhs, int semantics, const bg_ensemble_desc &d,
+                    hipStream_t stream);
+bool rk_supported(int method, int rhs, uint32_t dim);
+
+// Single-instance path production (one lane).  The state block lives in
+// device memory and is resumable: the kernel stops when the path buffer is
+// full and the host relaunches it after draining the buffer.
+struct PathState {
+    double t, dt, end, tol, dt_min, dt_max;
+    uint64_t attempts, max_attempts;
+    uint32_t n_accepted, n_rejected;
+    int32_t status;   // BG_OK while running / at Done, else the failure code
+    int32_t finished; // 1 once Done or failed
+    uint32_t count;   // entries written into the path buffer by this launch
+    uint32_t pad;
+    double y[128];
+    double K[6][128];
+    double p[8];
+};
+int rk_launch_path(int method, int rhs, int semantics, uint32_t dim, PathState *state,
+                   double *path_t, double *path_y, uint32_t cap, hipStream_t stream);
+
+int device_cu_count(int device);
+
+}  // namespace bg
+
+#define BG_HIP_CHECK(expr)                         \
+    do {                                           \
+        hipError_t _e = (expr);                    \
+        if (_e != hipSuccess) return BG_ERR_HIP;   \
+    } while (0)

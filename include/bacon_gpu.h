@@ -1,0 +1,209 @@
+/*
+ * bacon_gpu.h — C ABI of the MI355X-native batched IVP engine (libbacon_gpu.so).
+ *
+ * Plain pointers and sizes only.  Every entry point returns an int status:
+ * 0 = OK, 1..12 = the reference's IVPError variants in declaration order
+ * (src/ivp.rs:50-76), negative = misuse of this C interface or a HIP failure.
+ *
+ * Two families:
+ *   bg_*           single-instance builder + iterator mirroring the IVPSolver
+ *                  builder (src/ivp.rs:134-190) and IVPIterator
+ *                  (src/ivp.rs:192-239).  The whole path is produced on the
+ *                  GPU by one lane; bg_next walks it on the host.
+ *   bg_ensemble_*  the batched builder: the same setters taking per-instance
+ *                  arrays, solved by persistent gfx950 kernels.
+ * plus device-pointer entry points (bg_*_device) for callers whose inputs
+ * already live in HBM.
+ *
+ * Right-hand sides are registered device functions selected by id (the
+ * Derivative closure of src/ivp.rs:34-48 cannot cross a C ABI into GPU code);
+ * per-instance parameters replace the reference's UserData.
+ */
+#ifndef BACON_GPU_H
+#define BACON_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define BG_OK 0
+#define BG_MISSING_PARAMETERS 1          /* IVPError::MissingParameters       */
+#define BG_USER_ERROR 2                  /* IVPError::UserError               */
+#define BG_TOLERANCE_OOB 3               /* IVPError::ToleranceOOB            */
+#define BG_TIME_DELTA_OOB 4              /* IVPError::TimeDeltaOOB            */
+#define BG_TIME_END_OOB 5                /* IVPError::TimeEndOOB              */
+#define BG_TIME_START_OOB 6              /* IVPError::TimeStartOOB            */
+#define BG_FROM_PRIMITIVE_FAILURE 7      /* IVPError::FromPrimitiveFailure    */
+#define BG_MINIMUM_TIME_DELTA_EXCEEDED 8 /* IVPError::MinimumTimeDeltaExceeded */
+#define BG_MAXIMUM_ITERATIONS_EXCEEDED 9 /* IVPError::MaximumIterationsExceeded*/
+#define BG_SINGULAR_MATRIX 10            /* IVPError::SingularMatrix          */
+#define BG_DYNAMIC_ON_STATIC 11          /* IVPError::DynamicOnStatic         */
+#define BG_STATIC_ON_DYNAMIC 12          /* IVPError::StaticOnDynamic         */
+#define BG_DONE 100                      /* iterator exhausted (IVPStatus::Done) */
+#define BG_ERR_INVALID_ARGUMENT (-1)     /* null pointer, size mismatch, ...  */
+#define BG_ERR_HIP (-2)                  /* a HIP runtime call failed         */
+#define BG_ERR_UNSUPPORTED (-3)          /* method/RHS/dimension not compiled */
+#define BG_ERR_STATE (-4)                /* call out of order (e.g. next before solve) */
+
+/* ---- methods (coefficient plugins rk.rs:20-45 / bdf.rs:20-41) ---------- */
+#define BG_RK45 0 /* RungeKutta45, rk.rs:561 */
+#define BG_RK23 1 /* RungeKutta23, rk.rs:656 */
+#define BG_BDF6 2 /* BDF6, bdf.rs:706        */
+#define BG_BDF2 3 /* BDF2, bdf.rs:762        */
+
+/* ---- semantics: the reference's quirk register as data ------------------
+ * BG_AS_BUILT : the shipped behaviour (RK stage matrix read transposed,
+ *               FD Jacobian column (G+ + G-)/2dt, lower residual with the
+ *               higher history weights, f at t_n, bootstrap time restore
+ *               t -= dt - 7).
+ * BG_INTENDED : the documented methods (Fehlberg rows as rows, central
+ *               difference, own BDF5 weights, f at t_n+1, t -= 7 dt).
+ * Every other bacon semantic (safety factor 1.0, clamps, dt_min failure,
+ * RK4 bootstrap, Broyden corrector) is shared. */
+#define BG_AS_BUILT 0
+#define BG_INTENDED 1
+
+/* ---- dimension kinds (Dimension trait, lib.rs:52-76) -------------------- */
+#define BG_DIM_STATIC 0 /* nalgebra Const<D> */
+#define BG_DIM_DYN 1    /* nalgebra Dyn      */
+
+/* ---- registered right-hand sides (device functions) -------------------- */
+#define BG_RHS_EXP 0            /* y' = y (bdf.rs:769-771)                   */
+#define BG_RHS_NEG 1            /* y' = -y (bdf.rs:781-783)                  */
+#define BG_RHS_QUADRATIC 2      /* y' = -2t (rk.rs:664-666)                  */
+#define BG_RHS_COS 3            /* y' = cos t (rk.rs:668-670)                */
+#define BG_RHS_LORENZ 4         /* params: sigma, rho, beta; dim 3           */
+#define BG_RHS_LINEAR 5         /* y' = A y + b, shared [A row-major; b]     */
+#define BG_RHS_BRUSSELATOR 6    /* params: A, B, alpha; dim 2*cells          */
+#define BG_RHS_LOTKA_VOLTERRA 7 /* params: alpha, beta, gamma, delta; dim 2  */
+#define BG_RHS_OSCILLATOR 8     /* y0' = y1, y1' = -y0 (ivp.rs cos_deriv)    */
+#define BG_RHS_FAIL_AFTER 9     /* y' = y; UserError once t >= params[0]     */
+
+/* ======================================================================== */
+/* Single-instance builder + iterator                                       */
+/* ======================================================================== */
+typedef struct bg_solver bg_solver;
+
+/* RungeKutta45::new() / BDF6::new() (rk.rs:136-148, bdf.rs:144-156) for a
+ * solver whose dimension type is `dim_kind`; `dim` is the static size.
+ * A BG_DIM_DYN kind fails with BG_STATIC_ON_DYNAMIC like Dyn::dim(). */
+int bg_new(int method, int dim_kind, uint32_t dim, bg_solver **out);
+/* new_dyn(size) (rk.rs:150-162): a BG_DIM_STATIC kind fails with
+ * BG_DYNAMIC_ON_STATIC like Const::dim_dyn(). */
+int bg_new_dyn(int method, int dim_kind, uint32_t size, bg_solver **out);
+/* IVPSolver::dim (ivp.rs:166) */
+int bg_dim(const bg_solver *s, uint32_t *dim);
+/* setters: rk.rs:168-247 / bdf.rs:176-255, same validation and order */
+int bg_with_tolerance(bg_solver *s, double tol);
+int bg_with_maximum_dt(bg_solver *s, double max);
+int bg_with_minimum_dt(bg_solver *s, double min);
+int bg_with_initial_time(bg_solver *s, double initial);
+int bg_with_ending_time(bg_solver *s, double ending);
+/* with_initial_conditions_slice (ivp.rs:177-180); n must equal the dimension */
+int bg_with_initial_conditions_slice(bg_solver *s, const double *y0, size_t n);
+/* with_derivative (rk.rs:244-247): a registered device RHS + its parameters */
+int bg_with_derivative(bg_solver *s, int rhs, const double *params, size_t n_params);
+/* quirk table selection (default BG_INTENDED) */
+int bg_with_semantics(bg_solver *s, int semantics);
+/* README.md:33-38 spellings of the same setters */
+int bg_with_dt_min(bg_solver *s, double min);
+int bg_with_dt_max(bg_solver *s, double max);
+int bg_with_start(bg_solver *s, double initial);
+int bg_with_end(bg_solver *s, double ending);
+/* solve(data) (rk.rs:249-343): checks MissingParameters, then produces the
+ * path on the GPU. */
+int bg_solve(bg_solver *s);
+/* IVPIterator::next (ivp.rs:220-238): BG_OK with (t, y[dim]) filled,
+ * BG_DONE at the end, or the failure code once (then BG_DONE: fused). */
+int bg_next(bg_solver *s, double *t, double *y);
+/* IVPStepper::time (ivp.rs:124): time of the last yielded value */
+int bg_time(const bg_solver *s, double *t);
+void bg_destroy(bg_solver *s);
+
+/* ======================================================================== */
+/* Batched builder                                                          */
+/* ======================================================================== */
+typedef struct bg_ensemble bg_ensemble;
+
+/* counters per instance */
+typedef struct {
+    uint32_t n_accepted, n_rejected;
+} bg_counts;
+
+int bg_ensemble_new(int method, uint32_t dim, uint64_t n, bg_ensemble **out);
+/* per-instance setters: `count` is 1 (broadcast) or n.  The validation of
+ * each element is the single-instance setter's. */
+int bg_ensemble_with_tolerance(bg_ensemble *e, const double *tol, uint64_t count);
+int bg_ensemble_with_maximum_dt(bg_ensemble *e, const double *v, uint64_t count);
+int bg_ensemble_with_minimum_dt(bg_ensemble *e, const double *v, uint64_t count);
+int bg_ensemble_with_initial_time(bg_ensemble *e, const double *v, uint64_t count);
+int bg_ensemble_with_ending_time(bg_ensemble *e, const double *v, uint64_t count);
+/* y0 structure-of-arrays: y0[d*n + i]; count = dim*n */
+int bg_ensemble_with_initial_conditions(bg_ensemble *e, const double *y0, uint64_t count);
+/* params structure-of-arrays: params[k*n + i]; shared: RHS-wide data */
+int bg_ensemble_with_derivative(bg_ensemble *e, int rhs, const double *params,
+                                uint32_t n_params, const double *shared, uint64_t n_shared);
+int bg_ensemble_with_semantics(bg_ensemble *e, int semantics);
+/* cap on step attempts per instance (guards the reference's endless Redo on
+ * a NaN error); 0 = default */
+int bg_ensemble_with_max_attempts(bg_ensemble *e, uint64_t max_attempts);
+/* Solve on the GPU (synchronous). */
+int bg_ensemble_solve(bg_ensemble *e);
+/* Plain path: the same device step code launched once per instance,
+ * host-synchronous, one instance after another. */
+int bg_ensemble_solve_plain(bg_ensemble *e);
+/* Copy results to host arrays (any may be NULL); y_out is [dim][n]. */
+int bg_ensemble_results(bg_ensemble *e, double *t_out, double *y_out, int32_t *status,
+                        bg_counts *counts);
+void bg_ensemble_destroy(bg_ensemble *e);
+
+/* ======================================================================== */
+/* Device-pointer entry points (inputs already resident in HBM)             */
+/* ======================================================================== */
+
+/* Launch shape chosen by the autotuner (zeros = library defaults). */
+typedef struct {
+    uint32_t block;           /* threads per block (multiple of 64)          */
+    uint32_t blocks_per_cu;   /* persistent blocks per CU (0 = occupancy)    */
+    uint32_t refill_min;      /* refill a wave once this many lanes are idle */
+    uint32_t reserved;
+} bg_launch;
+
+typedef struct {
+    uint64_t n;
+    uint32_t dim, n_params;
+    /* inputs (device pointers; per-instance arrays may be NULL = scalar) */
+    const double *y0;      /* [dim][n]        */
+    const double *params;  /* [n_params][n]   */
+    const double *shared;  /* RHS-wide block  */
+    const double *t0, *t_end, *tol, *dt_min, *dt_max; /* [n] or NULL */
+    double t0_s, t_end_s, tol_s, dt_min_s, dt_max_s;  /* scalar fallbacks */
+    const int32_t *order;  /* schedule permutation [n] or NULL */
+    uint64_t max_attempts; /* 0 = default */
+    /* outputs (device pointers; any may be NULL) */
+    double *t_out;         /* [n]             */
+    double *y_out;         /* [dim][n]        */
+    int32_t *status;       /* [n]             */
+    bg_counts *counts;     /* [n]             */
+} bg_ensemble_desc;
+
+/* Batched RK (RK45 / RK23) in one persistent launch on `stream`
+ * (hipStream_t, NULL = default stream).  Asynchronous. */
+int bg_rk_solve_device(int method, int rhs, int semantics, const bg_ensemble_desc *d,
+                       const bg_launch *launch, void *stream);
+/* The plain per-instance path over the same descriptor (synchronous). */
+int bg_rk_solve_plain_device(int method, int rhs, int semantics, const bg_ensemble_desc *d,
+                             void *stream);
+
+/* Library / device information */
+int bg_version(void);
+int bg_device_info(int *n_cu, int *arch_major, int *arch_minor);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,240 @@
+"""GPU parity of the batched RK kernels (through the C ABI) against the oracle.
+
+The kernels perform exactly the operations of the oracle's DEVICE profile
+(explicit fma, sqrt(sqrt) for the controller exponent, all IEEE correctly
+rounded), so for right-hand sides without transcendental functions the bar
+is bit-exact: final time, state, status and accepted/rejected counts of every
+instance.  Right-hand sides calling cos() are compared within the
+reference's own tolerance.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+from bacon_amd import _lib as L
+from bacon_amd import ivp
+from bacon_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_ensemble(w, semantics, profile=O.PROFILE_DEVICE, max_attempts=0, method=O.RK45):
+    return O.solve_ensemble(method, w["rhs"], w["y0"], w["params"], w["t_end"], w["tol"],
+                            t0=w.get("t0", 0.0), dt_min=w["dt_min"], dt_max=w["dt_max"],
+                            semantics=semantics, profile=profile, threads=0,
+                            max_attempts=max_attempts)
+
+
+def assert_bit_exact(g, o):
+    assert np.array_equal(g["status"], o["status"])
+    assert np.array_equal(g["n_accepted"], o["n_accepted"])
+    assert np.array_equal(g["n_rejected"], o["n_rejected"])
+    assert np.array_equal(g["t"], o["t"])
+    assert np.array_equal(g["y"], o["y"])
+
+
+# ------------------------------------------------------------- reference tests
+def _iter_path(solver):
+    return solver.solve().collect_vec()
+
+
+@pytest.mark.parametrize("semantics", [L.AS_BUILT, L.INTENDED])
+@pytest.mark.parametrize("cls", [ivp.RungeKutta45, ivp.RungeKutta23])
+def test_reference_quadratic_through_abi(gpu, orc, cls, semantics):
+    # rk.rs:682-719 through the single-instance builder and iterator
+    s = (cls.new(1).with_minimum_dt(0.0001).with_maximum_dt(0.1)
+         .with_initial_time(0.0).with_ending_time(10.0).with_tolerance(1e-5)
+         .with_initial_conditions_slice([1.0]).with_derivative("quadratic")
+         .with_semantics(semantics))
+    path = _iter_path(s)
+    assert len(path) > 0
+    for t, y in path:
+        assert abs(y[0] - (1.0 - t * t)) <= 1e-4
+    method = O.RK45 if cls is ivp.RungeKutta45 else O.RK23
+    r = O.solve(method, "quadratic", [1.0], 0.0, 10.0, 1e-5, 1e-4, 0.1,
+                semantics=semantics, profile=O.PROFILE_DEVICE, path_cap=10000)
+    assert len(path) == r["path_len"]
+    assert np.array_equal(np.array([p[0] for p in path]), r["path_t"])
+    assert np.array_equal(np.array([p[1][0] for p in path]), r["path_y"][:, 0])
+
+
+@pytest.mark.parametrize("semantics", [L.AS_BUILT, L.INTENDED])
+@pytest.mark.parametrize("cls", [ivp.RungeKutta45, ivp.RungeKutta23])
+def test_reference_sine_through_abi(gpu, orc, cls, semantics):
+    # rk.rs:721-758
+    s = (cls.new(1).with_minimum_dt(0.001).with_maximum_dt(0.01).with_tolerance(0.0001)
+         .with_initial_time(0.0).with_ending_time(10.0)
+         .with_initial_conditions_slice([0.0]).with_derivative("cos")
+         .with_semantics(semantics))
+    path = _iter_path(s)
+    assert len(path) > 0
+    for t, y in path:
+        assert abs(y[0] - np.sin(t)) <= 0.01
+    method = O.RK45 if cls is ivp.RungeKutta45 else O.RK23
+    r = O.solve(method, "cos", [0.0], 0.0, 10.0, 1e-4, 1e-3, 1e-2, semantics=semantics,
+                profile=O.PROFILE_DEVICE)
+    assert abs(path[-1][1][0] - r["y"][0]) <= 10 * 1e-4
+
+
+def test_iterator_fuses_after_failure(gpu, orc):
+    s = (ivp.RK45.new(1).with_minimum_dt(1e-6).with_maximum_dt(0.01).with_tolerance(1e-8)
+         .with_initial_time(0.0).with_ending_time(2.0).with_initial_conditions([1.0])
+         .with_derivative("fail_after", [0.5]))
+    it = s.solve()
+    vals = []
+    with pytest.raises(ivp.IVPError) as e:
+        for v in it:
+            vals.append(v)
+    assert e.value.name == "UserError"
+    assert len(vals) > 0
+    assert list(it) == []  # fused (ivp.rs:232-235)
+    r = O.solve(O.RK45, "fail_after", [1.0], 0.0, 2.0, 1e-8, 1e-6, 0.01, params=[0.5],
+                profile=O.PROFILE_DEVICE, path_cap=100000)
+    assert len(vals) == r["path_len"]
+    assert np.array_equal(np.array([v[0] for v in vals]), r["path_t"])
+
+
+def test_min_dt_failure_as_built_single(gpu, orc):
+    w = W.lorenz_plain(1)
+    s = (ivp.RK45.new(3).with_minimum_dt(1e-6).with_maximum_dt(0.1).with_tolerance(1e-6)
+         .with_initial_time(0.0).with_ending_time(10.0)
+         .with_initial_conditions(w["y0"][:, 0]).with_derivative("lorenz", w["params"][:, 0])
+         .with_semantics(L.AS_BUILT))
+    with pytest.raises(ivp.IVPError) as e:
+        s.solve().collect_vec()
+    assert e.value.name == "MinimumTimeDeltaExceeded"
+
+
+# ------------------------------------------------------------- ensembles
+@pytest.mark.parametrize("semantics", [L.AS_BUILT, L.INTENDED])
+def test_plain_workload_bit_exact(gpu, orc, semantics):
+    """1024 Lorenz-63 instances, tol 1e-6: every instance bit-identical to the
+    oracle (as built: every instance fails with MinimumTimeDeltaExceeded)."""
+    w = W.lorenz_plain(1024)
+    g = ivp.solve_ensemble(L.RK45, w, semantics=semantics)
+    o = oracle_ensemble(w, semantics)
+    assert_bit_exact(g, o)
+    if semantics == L.AS_BUILT:
+        assert np.all(g["status"] == 8)
+    else:
+        assert np.all(g["status"] == 0)
+        # within 10x tol of the reference's own arithmetic (no fma, powf)
+        b = oracle_ensemble(w, semantics, profile=O.PROFILE_BACON)
+        assert np.abs(g["y"] - b["y"]).max() < 10 * 1e-6
+
+
+def test_batched_workload_subset_bit_exact(gpu, orc):
+    """Instances of the 2^20 batched workload (uneven t_end, tol 1e-10..1e-4)."""
+    w = W.lorenz_batched(2048)
+    g = ivp.solve_ensemble(L.RK45, w)
+    o = oracle_ensemble(w, L.INTENDED)
+    assert_bit_exact(g, o)
+    assert np.all(g["status"] == 0)
+
+
+def test_plain_path_equals_batched(gpu, orc):
+    w = W.lorenz_batched(48)
+    a = ivp.solve_ensemble(L.RK45, w)
+    b = ivp.solve_ensemble(L.RK45, w, plain=True)
+    assert_bit_exact(a, b)
+
+
+def test_lotka_volterra_and_rk23_bit_exact(gpu, orc):
+    w = W.lotka_volterra(512)
+    for method in (L.RK45, L.RK23):
+        g = ivp.solve_ensemble(method, w)
+        o = oracle_ensemble(w, L.INTENDED, method=method)
+        assert_bit_exact(g, o)
+
+
+def test_mixed_failures_do_not_affect_other_instances(gpu, orc):
+    w = W.lorenz_plain(256)
+    w["y0"][:, 7] = [1e200, -1e200, 1e200]  # NaN error -> attempt cap
+    w["tol"][11] = 1e-15                     # forces dt below dt_min
+    g = ivp.solve_ensemble(L.RK45, w, max_attempts=20000)
+    o = oracle_ensemble(w, L.INTENDED, max_attempts=20000)
+    assert g["status"][7] == 9 and g["status"][11] == 8
+    assert np.count_nonzero(g["status"]) == 2
+    assert np.array_equal(g["status"], o["status"])
+    ok = g["status"] == 0
+    assert np.array_equal(g["y"][:, ok], o["y"][:, ok])
+
+
+def test_user_error_per_instance(gpu, orc):
+    n = 64
+    y0 = np.ones((1, n))
+    params = np.linspace(0.1, 3.0, n)[None, :]
+    w = dict(rhs="fail_after", dim=1, y0=y0, params=params, t0=0.0,
+             t_end=np.full(n, 2.0), tol=np.full(n, 1e-8), dt_min=1e-9, dt_max=0.01)
+    g = ivp.solve_ensemble(L.RK45, w)
+    o = oracle_ensemble(w, L.INTENDED)
+    assert np.array_equal(g["status"], o["status"])
+    assert set(np.unique(g["status"])) == {0, 2}
+    assert_bit_exact(g, o)
+
+
+def test_empty_and_single(gpu, orc):
+    w = W.lorenz_plain(1)
+    g = ivp.solve_ensemble(L.RK45, w)
+    o = oracle_ensemble(w, L.INTENDED)
+    assert_bit_exact(g, o)
+    e = ivp.Ensemble(L.RK45, 3, 0)
+    (e.with_minimum_dt(1e-6).with_maximum_dt(0.1).with_tolerance(1e-6)
+     .with_initial_time(0.0).with_ending_time(1.0).with_initial_conditions(np.zeros(0))
+     .with_derivative("lorenz", np.zeros((3, 0))))
+    r = e.solve()
+    assert r["t"].size == 0
+
+
+def test_device_entry_with_order_and_done_instances(gpu, orc):
+    """Device-pointer entry point: inputs in HBM, a longest-first schedule
+    permutation, instances whose start is already at the end (Done before
+    the first step), and launch shapes from the tuning space."""
+    import torch
+    w = W.lorenz_batched(4096)
+    t0 = np.zeros(4096)
+    t0[::97] = w["t_end"][::97]  # Done immediately
+    dev = torch.device("cuda:0")
+    T = lambda a, dt=torch.float64: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)  # noqa: E731
+    y0, params = T(w["y0"]), T(w["params"])
+    tend, tol, t0d = T(w["t_end"]), T(w["tol"]), T(t0)
+    order = T(W.cost_order(w["t_end"], w["tol"]), torch.int32)
+    ref = None
+    for block, refill in ((256, 1), (64, 8), (128, 32)):
+        t_out = torch.zeros(4096, dtype=torch.float64, device=dev)
+        y_out = torch.zeros((3, 4096), dtype=torch.float64, device=dev)
+        st = torch.full((4096,), -7, dtype=torch.int32, device=dev)
+        cnt = torch.zeros((4096, 2), dtype=torch.int32, device=dev)
+        d = L.EnsembleDesc(n=4096, dim=3, n_params=3, y0=y0.data_ptr(),
+                           params=params.data_ptr(), t0=t0d.data_ptr(), t_end=tend.data_ptr(),
+                           tol=tol.data_ptr(), dt_min_s=1e-12, dt_max_s=0.1,
+                           order=order.data_ptr(), t_out=t_out.data_ptr(),
+                           y_out=y_out.data_ptr(), status=st.data_ptr(), counts=cnt.data_ptr())
+        lc = L.Launch(block=block, blocks_per_cu=0, refill_min=refill)
+        rc = L.lib().bg_rk_solve_device(L.RK45, L.RHS["lorenz"], L.INTENDED, C.byref(d),
+                                         C.byref(lc), None)
+        assert rc == 0
+        torch.cuda.synchronize()
+        got = dict(t=t_out.cpu().numpy(), y=y_out.cpu().numpy(), status=st.cpu().numpy(),
+                   n_accepted=cnt[:, 0].cpu().numpy().astype(np.int64),
+                   n_rejected=cnt[:, 1].cpu().numpy().astype(np.int64))
+        if ref is None:
+            w2 = dict(w)
+            w2["t0"] = 0.0
+            o = O.solve_ensemble(O.RK45, "lorenz", w["y0"], w["params"], w["t_end"],
+                                 w["tol"], dt_min=1e-12, dt_max=0.1, semantics=O.INTENDED,
+                                 profile=O.PROFILE_DEVICE)
+            done = t0 > 0
+            assert np.all(got["status"] == 0)
+            assert np.array_equal(got["t"][done], t0[done])
+            assert np.all(got["n_accepted"][done] == 0)
+            assert np.array_equal(got["y"][:, done], w["y0"][:, done])
+            nd = ~done
+            assert np.array_equal(got["y"][:, nd], o["y"][:, nd])
+            assert np.array_equal(got["n_accepted"][nd], o["n_accepted"][nd])
+            ref = got
+        else:
+            for k in ref:
+                assert np.array_equal(got[k], ref[k]), k
