@@ -192,7 +192,7 @@ static int solve_rk_path(bg_solver *s) {
     host.dt_min = s->dt_min;
     host.dt_max = s->dt_max;
     host.dt = (s->dt_max + s->dt_min) * 0.5;  // rk.rs:315
-    host.max_attempts = 50000000ull;
+    host.max_attempts = 10000000ull;
     for (uint32_t k = 0; k < D; ++k) host.y[k] = s->y0[k];
     for (size_t k = 0; k < s->params.size() && k < 8; ++k) host.p[k] = s->params[k];
     PathState *dstate = nullptr;

@@ -19,7 +19,7 @@
 
 namespace bg {
 
-static constexpr uint64_t kDefaultMaxAttempts = 50000000ull;
+static constexpr uint64_t kDefaultMaxAttempts = 10000000ull;
 static constexpr int64_t kNeedWork = -1;
 static constexpr int64_t kExhausted = -2;
 

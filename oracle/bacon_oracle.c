@@ -41,7 +41,7 @@
 #endif
 
 #define MAXD OR_MAX_DIM
-#define DEFAULT_MAX_ATTEMPTS 50000000LL
+#define DEFAULT_MAX_ATTEMPTS 10000000LL
 
 /* ------------------------------------------------------------------ */
 /* arithmetic profile helpers                                          */
