@@ -36,7 +36,7 @@ class Counts(C.Structure):
 
 class Launch(C.Structure):
     _fields_ = [("block", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-                ("refill_min", C.c_uint32), ("reserved", C.c_uint32)]
+                ("refill_min", C.c_uint32), ("prio_cutoff", C.c_uint32)]
 
 
 class EnsembleDesc(C.Structure):

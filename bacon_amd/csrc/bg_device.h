@@ -3,8 +3,9 @@
 // RK step attempt (bacon-sci src/ivp/rk.rs:361-423) for one lane.
 //
 // Operation order is fixed: every multiply-add is an explicit fma(), zero
-// tableau entries are skipped at compile time, x^(1/4) is sqrt(sqrt(x)).
-// Add, multiply, divide, sqrt and fma are correctly rounded on gfx950, so a
+// tableau entries are skipped at compile time, and the controller works on
+// squared norms with the exact-operation root8() below.
+// Add, multiply, divide and fma are correctly rounded on gfx950, so a
 // lane's arithmetic is reproducible bit for bit by a CPU that performs the
 // same operations in the same order.  The file is compiled with
 // -ffp-contract=off so the compiler adds no fusions of its own.
@@ -160,6 +161,34 @@ struct RhsOscillator {
 };
 
 // ---------------------------------------------------------------------------
+// r^(1/8) for r in (1e-8, 65536) from exact operations only (frexp, fma,
+// ldexp, products of correctly rounded constants): r = m 2^e, m in [1,2);
+// m^(1/8) by a degree-7 polynomial in (m - 1.5) interpolating at Chebyshev
+// nodes (relative error < 2.4e-8); 2^(e/8) = 2^floor(e/8) times a product of
+// 2^(1/8), 2^(1/4), 2^(1/2) selected by the bits of e mod 8.  NaN in, NaN
+// out.  Used as the step-size factor (tol/err)^(1/4).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double root8(double r) {
+    int ex;
+    const double mant = frexp(r, &ex);
+    const double x = mant * 2.0 - 1.5;
+    const int e = ex - 1;
+    double p = 0.0009511873810448949;
+    p = fma(p, x, -0.001688067994416299);
+    p = fma(p, x, 0.0026129162559138134);
+    p = fma(p, x, -0.005062886805578709);
+    p = fma(p, x, 0.010655080029744732);
+    p = fma(p, x, -0.02557132434659056);
+    p = fma(p, x, 0.0876657823239865);
+    p = fma(p, x, 1.0519895222336328);
+    const int a = e >> 3, b = e & 7;
+    const double s1 = (b & 1) ? 1.0905077326652577 : 1.0;
+    const double s2 = (b & 2) ? 1.189207115002721 : 1.0;
+    const double s4 = (b & 4) ? 1.4142135623730951 : 1.0;
+    return ldexp(p * ((s1 * s2) * s4), a);
+}
+
+// ---------------------------------------------------------------------------
 // One lane's RK state and one call of RungeKuttaSolver::step.
 // ---------------------------------------------------------------------------
 enum : int { STEP_ACCEPTED = -1, STEP_REJECTED = -2 };
@@ -213,9 +242,25 @@ struct RkLane {
         double ss = E[0] * E[0];
 #pragma unroll
         for (int d = 1; d < D; ++d) ss = fma(E[d], E[d], ss);
-        const double error = sqrt(ss) / dt;
+        // rk.rs:390-412 in squared form: err = |E|/dt <= tol  <=>  ss <= q^2
+        // with q = tol*dt; delta = (tol/err)^(1/4) = r^(1/8), r = q^2/ss,
+        // and the clamps delta <= 0.1 / delta >= 4 become r <= 1e-8 /
+        // r >= 65536 (safety factor 100/100 = 1.0, rk.rs:266-268).
+        const double q = tol * dt;
+        const double q2 = q * q;
+        const bool accepted = ss <= q2;
+        const double r = q2 / ss;
+        const double f0 = r <= 1e-8 ? 0.1 : (r >= 65536.0 ? 4.0 : root8(r));
+        // near r = 1 the sign of delta - 1 ~ (r - 1)/8 is far below root8's
+        // error: bound by the tangent 1 + (r - 1)/8 (r^(1/8) is concave, the
+        // tangent is exact to second order at 1), then keep the exact power's
+        // side of 1 (a rejection shrinks dt, an acceptance never does); NaN
+        // passes through
+        const double tang = fma(r - 1.0, 0.125, 1.0);
+        const double f = f0 < tang ? f0 : tang;
+        const double factor = accepted ? (f < 1.0 ? 1.0 : f)
+                                       : (f > 0x1.fffffffffffffp-1 ? 0x1.fffffffffffffp-1 : f);
         // rk.rs:392-398
-        const bool accepted = error <= tol;
         if (accepted) {
             t += dt;
 #pragma unroll
@@ -227,14 +272,7 @@ struct RkLane {
                 }
             }
         }
-        // rk.rs:400-412 (safety factor 100/100 = 1.0)
-        const double delta = sqrt(sqrt(tol / error));
-        if (delta <= 0.1)
-            dt *= 0.1;
-        else if (delta >= 4.0)
-            dt *= 4.0;
-        else
-            dt *= delta;
+        dt *= factor;
         if (dt > dt_max) dt = dt_max;
         // rk.rs:414-416
         if (dt < dt_min && t < end) return BG_MINIMUM_TIME_DELTA_EXCEEDED;

@@ -71,7 +71,7 @@ __device__ __forceinline__ void rk_store(const RkLane<RHS, Tab> &L, const bg_ens
 // One attempt plus the iterator's bookkeeping.  Returns -1 to continue, or
 // the final status (BG_OK at Done).
 template <class RHS, class Tab>
-__device__ __forceinline__ int rk_advance(RkLane<RHS, Tab> &L, uint64_t &attempts, uint64_t cap,
+__device__ __forceinline__ int rk_advance(RkLane<RHS, Tab> &L, uint32_t &attempts, uint32_t cap,
                                           uint32_t &nacc, uint32_t &nrej) {
     if (attempts >= cap) return BG_MAXIMUM_ITERATIONS_EXCEEDED;
     const int st = L.attempt();
@@ -88,51 +88,62 @@ __device__ __forceinline__ int rk_advance(RkLane<RHS, Tab> &L, uint64_t &attempt
     return st;
 }
 
+// Persistent ensemble kernel.  Outer loop: refill idle lanes (one atomicAdd
+// per wave), store finished instances.  Inner loop: step attempts until at
+// least `refill_min` lanes of the wave have finished (or all have).  Waves
+// holding an instance among the first `prio_cutoff` schedule positions run at
+// raised wave priority so the longest solves are not starved.
 template <class RHS, class Tab>
 __global__ void __launch_bounds__(256)
     rk_ensemble_kernel(const bg_ensemble_desc d, uint32_t *__restrict__ counter,
-                       uint32_t refill_min) {
+                       uint32_t refill_min, uint32_t prio_cutoff) {
     RkLane<RHS, Tab> L;
     const int lane = __lane_id();
-    const uint64_t cap = d.max_attempts ? d.max_attempts : kDefaultMaxAttempts;
+    const uint32_t cap =
+        (uint32_t)((d.max_attempts && d.max_attempts < kDefaultMaxAttempts) ? d.max_attempts
+                                                                           : kDefaultMaxAttempts);
     int64_t inst = kNeedWork;
-    uint64_t attempts = 0;
-    uint32_t nacc = 0, nrej = 0;
+    uint64_t pos = 0;
+    uint32_t attempts = 0, nacc = 0, nrej = 0;
     for (;;) {
+        // ---- refill --------------------------------------------------------
         const uint64_t need = __ballot(inst == kNeedWork);
-        const uint64_t busy = __ballot(inst >= 0);
-        if (need != 0 && (busy == 0 || __popcll(need) >= (int)refill_min)) {
+        if (need != 0) {
             const int leader = __ffsll((unsigned long long)need) - 1;
             uint32_t base = 0;
             if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(need));
             base = __shfl(base, leader);
             if (inst == kNeedWork) {
                 const uint64_t below = need & ((1ull << lane) - 1ull);
-                const uint64_t k = (uint64_t)base + (uint64_t)__popcll(below);
-                if (k < d.n) {
-                    inst = d.order ? (int64_t)d.order[k] : (int64_t)k;
+                pos = (uint64_t)base + (uint64_t)__popcll(below);
+                if (pos < d.n) {
+                    inst = d.order ? (int64_t)d.order[pos] : (int64_t)pos;
                     rk_load(L, d, (uint64_t)inst);
-                    attempts = 0;
-                    nacc = nrej = 0;
-                    if (L.t >= L.end) {  // Done before the first step
-                        rk_store(L, d, (uint64_t)inst, BG_OK, 0, 0);
-                        inst = kNeedWork;
-                    }
+                    attempts = nacc = nrej = 0;
                 } else {
                     inst = kExhausted;
                 }
             }
-        }
-        if (__ballot(inst >= 0) == 0) {
-            if (__ballot(inst == kNeedWork) == 0) break;
-            continue;
-        }
-        if (inst >= 0) {
-            const int st = rk_advance(L, attempts, cap, nacc, nrej);
-            if (st >= 0) {
-                rk_store(L, d, (uint64_t)inst, st, nacc, nrej);
-                inst = kNeedWork;
+            if (prio_cutoff) {
+                if (__ballot(inst >= 0 && pos < prio_cutoff) != 0)
+                    __builtin_amdgcn_s_setprio(2);
+                else
+                    __builtin_amdgcn_s_setprio(0);
             }
+        }
+        if (__ballot(inst >= 0) == 0) break;
+        // ---- attempts ------------------------------------------------------
+        const uint64_t active = __ballot(inst >= 0);
+        const int want = min((int)refill_min, __popcll(active));
+        int st = (inst >= 0 && L.t >= L.end) ? BG_OK : -1;  // Done before any step
+        for (;;) {
+            const uint64_t done = __ballot(inst >= 0 && st >= 0);
+            if (__popcll(done) >= want) break;
+            if (inst >= 0 && st < 0) st = rk_advance(L, attempts, cap, nacc, nrej);
+        }
+        if (inst >= 0 && st >= 0) {
+            rk_store(L, d, (uint64_t)inst, st, nacc, nrej);
+            inst = kNeedWork;
         }
     }
 }
@@ -143,9 +154,11 @@ __global__ void __launch_bounds__(64) rk_plain_kernel(const bg_ensemble_desc d, 
     if (threadIdx.x != 0) return;
     RkLane<RHS, Tab> L;
     const uint64_t i = d.order ? (uint64_t)d.order[k] : k;
-    const uint64_t cap = d.max_attempts ? d.max_attempts : kDefaultMaxAttempts;
+    const uint32_t cap =
+        (uint32_t)((d.max_attempts && d.max_attempts < kDefaultMaxAttempts) ? d.max_attempts
+                                                                           : kDefaultMaxAttempts);
     rk_load(L, d, i);
-    uint64_t attempts = 0;
+    uint32_t attempts = 0;
     uint32_t nacc = 0, nrej = 0;
     int st = L.t >= L.end ? BG_OK : -1;
     while (st < 0) st = rk_advance(L, attempts, cap, nacc, nrej);
@@ -175,7 +188,8 @@ __global__ void __launch_bounds__(64)
         for (int k = 0; k < D; ++k) L.K[s][k] = st->K[s][k];
 #pragma unroll
     for (int k = 0; k < RHS::P; ++k) L.p[k] = st->p[k];
-    uint64_t attempts = st->attempts;
+    uint32_t attempts = (uint32_t)st->attempts;
+    const uint32_t max_att = (uint32_t)st->max_attempts;
     uint32_t nacc = st->n_accepted, nrej = st->n_rejected, count = 0;
     int status = BG_OK, finished = st->finished;
     while (!finished && count < cap) {
@@ -184,7 +198,7 @@ __global__ void __launch_bounds__(64)
             break;
         }
         const uint32_t before = nacc;
-        const int r = rk_advance(L, attempts, st->max_attempts, nacc, nrej);
+        const int r = rk_advance(L, attempts, max_att, nacc, nrej);
         if (nacc != before) {
             path_t[count] = L.t;
 #pragma unroll
@@ -291,7 +305,8 @@ int rk_launch_ensemble(int method, int rhs, int semantics, const bg_ensemble_des
         BG_HIP_CHECK(hipMallocAsync((void **)&counter, 64, stream));
         zero_u32_kernel<<<1, 64, 0, stream>>>(counter);
         const uint32_t refill_min = lc.refill_min ? lc.refill_min : 1u;
-        rk_ensemble_kernel<R, T><<<(uint32_t)grid, block, 0, stream>>>(d, counter, refill_min);
+        rk_ensemble_kernel<R, T>
+            <<<(uint32_t)grid, block, 0, stream>>>(d, counter, refill_min, lc.prio_cutoff);
         BG_HIP_CHECK(hipGetLastError());
         BG_HIP_CHECK(hipFreeAsync(counter, stream));
         return BG_OK;

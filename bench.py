@@ -54,6 +54,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="before the solve, run a plain device copy of this many MiB "
+                         "(reference dispatch for HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -106,6 +109,15 @@ def main():
         if rc != 0:
             raise RuntimeError(f"bg_rk_solve_device returned {rc}")
 
+    if args.copy_calibration_mib:
+        nb = args.copy_calibration_mib << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy calibration: {nb} bytes read, {nb} bytes written")
+        del src, dst
+
     log(f"[bench] warmup {args.warmup}")
     for _ in range(args.warmup):
         run_once()
@@ -132,6 +144,7 @@ def main():
     n_rej = int(cnt[:, 1].sum())
     attempts = n_acc + n_rej
     n_fail = int(np.count_nonzero(st))
+    per_inst = cnt[:, 0] + cnt[:, 1]
     solves_per_s = n / (ms_per_step / 1000.0)
     avg_kernel_s = float(np.mean(kernel_ms)) / 1000.0
     flops = attempts * FLOPS_PER_ATTEMPT
@@ -160,6 +173,9 @@ def main():
         },
         "accepted_steps_per_s": n_acc / (ms_per_step / 1000.0),
         "attempts_per_launch": attempts,
+        "attempts_per_instance": {"mean": float(per_inst.mean()),
+                                  "p99": float(np.percentile(per_inst, 99)),
+                                  "max": int(per_inst.max())},
         "failed_instances": n_fail,
         "roofline": {
             "bound": "mfma",

@@ -170,7 +170,8 @@ typedef struct {
     uint32_t block;           /* threads per block (multiple of 64)          */
     uint32_t blocks_per_cu;   /* persistent blocks per CU (0 = occupancy)    */
     uint32_t refill_min;      /* refill a wave once this many lanes are idle */
-    uint32_t reserved;
+    uint32_t prio_cutoff;     /* waves holding one of the first prio_cutoff
+                                 scheduled instances run at raised priority */
 } bg_launch;
 
 typedef struct {

@@ -72,15 +72,35 @@ static double nal_norm(int n, const double *x) {
     return sqrt(res);
 }
 
-/* Device-order Euclidean norm: sequential fused accumulation. */
-static double dev_norm(int n, const double *x) {
+/* Device-order sum of squares: sequential fused accumulation. */
+static double dev_sumsq(int n, const double *x) {
     double s = x[0] * x[0];
     for (int i = 1; i < n; ++i) s = fma(x[i], x[i], s);
-    return sqrt(s);
+    return s;
 }
 
-static inline double prof_norm(int prof, int n, const double *x) {
-    return prof == OR_PROFILE_DEVICE ? dev_norm(n, x) : nal_norm(n, x);
+/* r^(1/8) for r in (1e-8, 65536), built from exact operations so that CPU and
+ * GPU agree bit for bit: r = m 2^e with m in [1,2) (frexp), m^(1/8) from a
+ * degree-7 polynomial in (m - 1.5) (Chebyshev-node interpolant, relative
+ * error < 2.4e-8), 2^(e/8) = 2^floor(e/8) * 2^((e mod 8)/8) with the second
+ * factor a product of 2^(1/8), 2^(1/4), 2^(1/2) selected by the bits of
+ * e mod 8.  A NaN argument returns NaN. */
+static double dev_root8(double r) {
+    static const double C[8] = {1.0519895222336328, 0.0876657823239865,
+                                -0.02557132434659056, 0.010655080029744732,
+                                -0.005062886805578709, 0.0026129162559138134,
+                                -0.001688067994416299, 0.0009511873810448949};
+    int ex = 0;
+    const double mant = frexp(r, &ex); /* r = mant 2^ex, mant in [0.5, 1) */
+    const double x = mant * 2.0 - 1.5;
+    const int e = ex - 1;
+    double p = C[7];
+    for (int k = 6; k >= 0; --k) p = fma(p, x, C[k]);
+    const int a = e >> 3, b = e & 7;
+    const double s1 = (b & 1) ? 1.0905077326652577 : 1.0;
+    const double s2 = (b & 2) ? 1.189207115002721 : 1.0;
+    const double s4 = (b & 4) ? 1.4142135623730951 : 1.0;
+    return ldexp(p * ((s1 * s2) * s4), a);
 }
 
 /* ------------------------------------------------------------------ */
@@ -387,10 +407,40 @@ static int rk_step(rk_solver *s, double *used_dt) {
         for (int j = 1; j < S; ++j)
             for (int d = 0; d < D; ++d) scratch[d] += s->K[j][d] * s->tab.e[j];
     }
-    const double error = prof_norm(prof, D, scratch) / s->dt;
+    /* rk.rs:390-412.  BACON: err = |E|/dt, accept err <= tol,
+     * delta = 1.0 * (tol/err)^(1/4), clamps 0.1 / 4.  DEVICE: the same
+     * decisions from ss = |E|^2 and q = tol*dt: accept ss <= q*q, and
+     * delta = (q*q/ss)^(1/8) with the clamps applied to r = q*q/ss
+     * (delta <= 0.1 <=> r <= 1e-8, delta >= 4 <=> r >= 65536); the root is
+     * dev_root8 above.  Both profiles keep the reference's branch structure. */
+    int accepted;
+    double factor;
+    if (prof == OR_PROFILE_DEVICE) {
+        const double ss = dev_sumsq(D, scratch);
+        const double q = s->tol * s->dt;
+        const double q2 = q * q;
+        accepted = ss <= q2;
+        const double r = q2 / ss;
+        const double f0 = r <= 1e-8 ? s->one_tenth : (r >= 65536.0 ? s->four : dev_root8(r));
+        /* Near r = 1 the controller's decisions hinge on the sign of
+         * delta - 1 = (r - 1)/8 + O((r-1)^2), far below the polynomial's
+         * 2.4e-8 error.  r^(1/8) is concave, so its tangent at 1,
+         * 1 + (r - 1)/8, bounds it from above and is exact to second order
+         * there: take the smaller of the two.  Then keep the exact power's
+         * side of 1 (delta < 1 on a rejection, >= 1 on an acceptance) against
+         * the final rounding.  NaN passes through. */
+        const double tang = fma(r - 1.0, 0.125, 1.0);
+        const double f = f0 < tang ? f0 : tang;
+        factor = accepted ? (f < 1.0 ? 1.0 : f) : (f > 0x1.fffffffffffffp-1 ? 0x1.fffffffffffffp-1 : f);
+    } else {
+        const double error = nal_norm(D, scratch) / s->dt;
+        accepted = error <= s->tol;
+        const double delta =
+            s->point_eighty_four * pow(s->tol / error, s->one_fourth);
+        factor = delta <= s->one_tenth ? s->one_tenth : (delta >= s->four ? s->four : delta);
+    }
 
     /* rk.rs:392-398: the 4th-order weights propagate the state */
-    const int accepted = error <= s->tol;
     if (accepted) {
         s->time += s->dt;
         for (int j = 0; j < S; ++j) {
@@ -413,16 +463,7 @@ static int rk_step(rk_solver *s, double *used_dt) {
     }
 
     /* rk.rs:400-412 */
-    const double ratio = s->tol / error;
-    const double delta =
-        s->point_eighty_four *
-        (prof == OR_PROFILE_DEVICE ? sqrt(sqrt(ratio)) : pow(ratio, s->one_fourth));
-    if (delta <= s->one_tenth)
-        s->dt *= s->one_tenth;
-    else if (delta >= s->four)
-        s->dt *= s->four;
-    else
-        s->dt *= delta;
+    s->dt *= factor;
     if (s->dt > s->dt_max) s->dt = s->dt_max;
 
     /* rk.rs:414-416 (Q8: an accepted step can still fail here) */
