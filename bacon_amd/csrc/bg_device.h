@@ -163,24 +163,23 @@ struct RhsOscillator {
 // ---------------------------------------------------------------------------
 // r^(1/8) for r in (1e-8, 65536) from exact operations only (frexp, fma,
 // ldexp, products of correctly rounded constants): r = m 2^e, m in [1,2);
-// m^(1/8) by a degree-7 polynomial in (m - 1.5) interpolating at Chebyshev
-// nodes (relative error < 2.4e-8); 2^(e/8) = 2^floor(e/8) times a product of
+// m^(1/8) by a degree-5 polynomial in (m - 1.5) interpolating at Chebyshev
+// nodes (relative error < 1.2e-6); 2^(e/8) = 2^floor(e/8) times a product of
 // 2^(1/8), 2^(1/4), 2^(1/2) selected by the bits of e mod 8.  NaN in, NaN
-// out.  Used as the step-size factor (tol/err)^(1/4).
+// out.  Used as the step-size factor (tol/err)^(1/4); the caller bounds it
+// by the tangent at r = 1.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double root8(double r) {
     int ex;
     const double mant = frexp(r, &ex);
     const double x = mant * 2.0 - 1.5;
     const int e = ex - 1;
-    double p = 0.0009511873810448949;
-    p = fma(p, x, -0.001688067994416299);
-    p = fma(p, x, 0.0026129162559138134);
-    p = fma(p, x, -0.005062886805578709);
-    p = fma(p, x, 0.010655080029744732);
-    p = fma(p, x, -0.02557132434659056);
-    p = fma(p, x, 0.0876657823239865);
-    p = fma(p, x, 1.0519895222336328);
+    double p = 0.0029682975037312034;
+    p = fma(p, x, -0.005693672682686368);
+    p = fma(p, x, 0.010621966427993313);
+    p = fma(p, x, -0.025512534910039072);
+    p = fma(p, x, 0.08766623673998887);
+    p = fma(p, x, 1.0519887150881322);
     const int a = e >> 3, b = e & 7;
     const double s1 = (b & 1) ? 1.0905077326652577 : 1.0;
     const double s2 = (b & 2) ? 1.189207115002721 : 1.0;

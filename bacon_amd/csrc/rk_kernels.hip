@@ -68,6 +68,37 @@ __device__ __forceinline__ void rk_store(const RkLane<RHS, Tab> &L, const bg_ens
     }
 }
 
+// Descriptor reads through a volatile pointer to a device-memory copy: each
+// field is a fresh load at the point of use, so the persistent kernel does not
+// keep ~30 descriptor SGPRs live across the attempt loop.
+__device__ __forceinline__ bg_ensemble_desc desc_inputs(const volatile bg_ensemble_desc *dv) {
+    bg_ensemble_desc d;
+    d.n = dv->n;
+    d.y0 = dv->y0;
+    d.params = dv->params;
+    d.t0 = dv->t0;
+    d.t_end = dv->t_end;
+    d.tol = dv->tol;
+    d.dt_min = dv->dt_min;
+    d.dt_max = dv->dt_max;
+    d.t0_s = dv->t0_s;
+    d.t_end_s = dv->t_end_s;
+    d.tol_s = dv->tol_s;
+    d.dt_min_s = dv->dt_min_s;
+    d.dt_max_s = dv->dt_max_s;
+    return d;
+}
+
+__device__ __forceinline__ bg_ensemble_desc desc_outputs(const volatile bg_ensemble_desc *dv) {
+    bg_ensemble_desc d;
+    d.n = dv->n;
+    d.t_out = dv->t_out;
+    d.y_out = dv->y_out;
+    d.status = dv->status;
+    d.counts = dv->counts;
+    return d;
+}
+
 // One attempt plus the iterator's bookkeeping.  Returns -1 to continue, or
 // the final status (BG_OK at Done).
 template <class RHS, class Tab>
@@ -95,13 +126,13 @@ __device__ __forceinline__ int rk_advance(RkLane<RHS, Tab> &L, uint32_t &attempt
 // raised wave priority so the longest solves are not starved.
 template <class RHS, class Tab>
 __global__ void __launch_bounds__(256)
-    rk_ensemble_kernel(const bg_ensemble_desc d, uint32_t *__restrict__ counter,
+    rk_ensemble_kernel(const volatile bg_ensemble_desc *dv, uint32_t *__restrict__ counter,
                        uint32_t refill_min, uint32_t prio_cutoff) {
     RkLane<RHS, Tab> L;
     const int lane = __lane_id();
-    const uint32_t cap =
-        (uint32_t)((d.max_attempts && d.max_attempts < kDefaultMaxAttempts) ? d.max_attempts
-                                                                           : kDefaultMaxAttempts);
+    const uint64_t max_att = dv->max_attempts;
+    const uint32_t cap = (uint32_t)((max_att && max_att < kDefaultMaxAttempts) ? max_att
+                                                                                : kDefaultMaxAttempts);
     int64_t inst = kNeedWork;
     uint64_t pos = 0;
     uint32_t attempts = 0, nacc = 0, nrej = 0;
@@ -116,9 +147,11 @@ __global__ void __launch_bounds__(256)
             if (inst == kNeedWork) {
                 const uint64_t below = need & ((1ull << lane) - 1ull);
                 pos = (uint64_t)base + (uint64_t)__popcll(below);
-                if (pos < d.n) {
-                    inst = d.order ? (int64_t)d.order[pos] : (int64_t)pos;
-                    rk_load(L, d, (uint64_t)inst);
+                const uint64_t n = dv->n;
+                if (pos < n) {
+                    const int32_t *order = dv->order;
+                    inst = order ? (int64_t)order[pos] : (int64_t)pos;
+                    rk_load(L, desc_inputs(dv), (uint64_t)inst);
                     attempts = nacc = nrej = 0;
                 } else {
                     inst = kExhausted;
@@ -142,7 +175,7 @@ __global__ void __launch_bounds__(256)
             if (inst >= 0 && st < 0) st = rk_advance(L, attempts, cap, nacc, nrej);
         }
         if (inst >= 0 && st >= 0) {
-            rk_store(L, d, (uint64_t)inst, st, nacc, nrej);
+            rk_store(L, desc_outputs(dv), (uint64_t)inst, st, nacc, nrej);
             inst = kNeedWork;
         }
     }
@@ -228,8 +261,13 @@ __global__ void __launch_bounds__(64)
     st->count = count;
 }
 
-__global__ void zero_u32_kernel(uint32_t *p) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0u;
+// Work-queue reset and descriptor copy into the per-launch workspace
+// (a kernel rather than memset/memcpy so the launch sequence stays capturable).
+__global__ void rk_setup_kernel(uint32_t *counter, bg_ensemble_desc *dst, const bg_ensemble_desc d) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *counter = 0u;
+        *dst = d;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -301,14 +339,17 @@ int rk_launch_ensemble(int method, int rhs, int semantics, const bg_ensemble_des
         uint64_t grid = (uint64_t)per_cu * (uint64_t)device_cu_count(dev);
         const uint64_t need = (d.n + block - 1) / block;
         if (grid > need) grid = need;
-        uint32_t *counter = nullptr;
-        BG_HIP_CHECK(hipMallocAsync((void **)&counter, 64, stream));
-        zero_u32_kernel<<<1, 64, 0, stream>>>(counter);
+        // workspace: [0,256) work counter, [256, 256+sizeof desc) descriptor copy
+        char *ws = nullptr;
+        BG_HIP_CHECK(hipMallocAsync((void **)&ws, 256 + sizeof(bg_ensemble_desc), stream));
+        uint32_t *counter = (uint32_t *)ws;
+        bg_ensemble_desc *ddesc = (bg_ensemble_desc *)(ws + 256);
+        rk_setup_kernel<<<1, 64, 0, stream>>>(counter, ddesc, d);
         const uint32_t refill_min = lc.refill_min ? lc.refill_min : 1u;
         rk_ensemble_kernel<R, T>
-            <<<(uint32_t)grid, block, 0, stream>>>(d, counter, refill_min, lc.prio_cutoff);
+            <<<(uint32_t)grid, block, 0, stream>>>(ddesc, counter, refill_min, lc.prio_cutoff);
         BG_HIP_CHECK(hipGetLastError());
-        BG_HIP_CHECK(hipFreeAsync(counter, stream));
+        BG_HIP_CHECK(hipFreeAsync(ws, stream));
         return BG_OK;
     });
 }

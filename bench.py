@@ -36,6 +36,10 @@ FLOPS_PER_ATTEMPT = 224
 # HBM bytes per instance: loads y0(3) rho sigma beta t_end tol (+order) and
 # stores t, y(3), status, counts
 BYTES_PER_INSTANCE = 8 * 8 + 4 + 8 * 4 + 4 + 8
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE and WRITE_SIZE (each
+# scaled by a 1 GiB copy-kernel calibration); see
+# profiles/rk_ensemble_lorenz_2p20.md
+MEASURED_TRAFFIC_BYTES = 1.054e9 + 0.238e9
 
 
 def log(msg):
@@ -185,7 +189,7 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / PEAK_FP64_TFLOPS,
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC_BYTES,
             "kernel": "rk_ensemble_kernel<RhsLorenz, Tableau<Fehlberg45,false>>",
             "flops_per_attempt": FLOPS_PER_ATTEMPT,
             "avg_kernel_ms": float(np.mean(kernel_ms)),

@@ -81,21 +81,21 @@ static double dev_sumsq(int n, const double *x) {
 
 /* r^(1/8) for r in (1e-8, 65536), built from exact operations so that CPU and
  * GPU agree bit for bit: r = m 2^e with m in [1,2) (frexp), m^(1/8) from a
- * degree-7 polynomial in (m - 1.5) (Chebyshev-node interpolant, relative
- * error < 2.4e-8), 2^(e/8) = 2^floor(e/8) * 2^((e mod 8)/8) with the second
+ * degree-5 polynomial in (m - 1.5) (Chebyshev-node interpolant, relative
+ * error < 1.2e-6), 2^(e/8) = 2^floor(e/8) * 2^((e mod 8)/8) with the second
  * factor a product of 2^(1/8), 2^(1/4), 2^(1/2) selected by the bits of
- * e mod 8.  A NaN argument returns NaN. */
+ * e mod 8.  A NaN argument returns NaN.  The caller bounds the result by the
+ * tangent at r = 1, which decides the controller near r = 1. */
 static double dev_root8(double r) {
-    static const double C[8] = {1.0519895222336328, 0.0876657823239865,
-                                -0.02557132434659056, 0.010655080029744732,
-                                -0.005062886805578709, 0.0026129162559138134,
-                                -0.001688067994416299, 0.0009511873810448949};
+    static const double C[6] = {1.0519887150881322, 0.08766623673998887,
+                                -0.025512534910039072, 0.010621966427993313,
+                                -0.005693672682686368, 0.0029682975037312034};
     int ex = 0;
     const double mant = frexp(r, &ex); /* r = mant 2^ex, mant in [0.5, 1) */
     const double x = mant * 2.0 - 1.5;
     const int e = ex - 1;
-    double p = C[7];
-    for (int k = 6; k >= 0; --k) p = fma(p, x, C[k]);
+    double p = C[5];
+    for (int k = 4; k >= 0; --k) p = fma(p, x, C[k]);
     const int a = e >> 3, b = e & 7;
     const double s1 = (b & 1) ? 1.0905077326652577 : 1.0;
     const double s2 = (b & 2) ? 1.189207115002721 : 1.0;
