@@ -49,7 +49,7 @@ class EnsembleDesc(C.Structure):
         ("dt_min_s", C.c_double), ("dt_max_s", C.c_double),
         ("order", C.c_void_p), ("max_attempts", C.c_uint64),
         ("t_out", C.c_void_p), ("y_out", C.c_void_p), ("status", C.c_void_p),
-        ("counts", C.c_void_p),
+        ("counts", C.c_void_p), ("work", C.c_void_p),
     ]
 
 
@@ -85,15 +85,19 @@ SIGNATURES = [
      [C.c_void_p, C.c_int, _dp, C.c_uint32, _dp, C.c_uint64]),
     ("bg_ensemble_with_semantics", C.c_int, [C.c_void_p, C.c_int]),
     ("bg_ensemble_with_max_attempts", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("bg_ensemble_with_lu_precision", C.c_int, [C.c_void_p, C.c_int]),
     ("bg_ensemble_solve", C.c_int, [C.c_void_p]),
     ("bg_ensemble_solve_plain", C.c_int, [C.c_void_p]),
     ("bg_ensemble_results", C.c_int,
      [C.c_void_p, _dp, _dp, C.POINTER(C.c_int32), C.POINTER(Counts)]),
+    ("bg_ensemble_work", C.c_int, [C.c_void_p, C.POINTER(Counts)]),
     ("bg_ensemble_destroy", None, [C.c_void_p]),
     ("bg_rk_solve_device", C.c_int,
      [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.POINTER(Launch), C.c_void_p]),
     ("bg_rk_solve_plain_device", C.c_int,
      [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.c_void_p]),
+    ("bg_bdf_solve_device", C.c_int,
+     [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.c_void_p]),
     ("bg_version", C.c_int, []),
     ("bg_device_info", C.c_int,
      [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),

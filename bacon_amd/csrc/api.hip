@@ -235,6 +235,59 @@ static int solve_rk_path(bg_solver *s) {
     return rc;
 }
 
+static int solve_bdf_path(bg_solver *s) {
+    if (!bdf_supported(s->method, s->rhs, s->dim)) return BG_ERR_UNSUPPORTED;
+    const uint32_t cap = 1u << 16;
+    const uint32_t D = s->dim;
+    BdfPathState host;
+    memset(&host, 0, sizeof host);
+    host.t = s->t0;
+    host.end = s->t_end;
+    host.tol = s->tol;
+    host.dt_min = s->dt_min;
+    host.dt_max = s->dt_max;
+    host.dt = (s->dt_max + s->dt_min) * 0.5;  // bdf.rs:302
+    host.max_steps = 1000000ull;
+    for (uint32_t k = 0; k < D; ++k) host.y[k] = host.save[k] = s->y0[k];
+    for (size_t k = 0; k < s->params.size() && k < 8; ++k) host.p[k] = s->params[k];
+    BdfPathState *dstate = nullptr;
+    double *dpt = nullptr, *dpy = nullptr;
+    int rc = BG_OK;
+    if (hipMalloc((void **)&dstate, sizeof(BdfPathState)) != hipSuccess ||
+        hipMalloc((void **)&dpt, sizeof(double) * cap) != hipSuccess ||
+        hipMalloc((void **)&dpy, sizeof(double) * cap * D) != hipSuccess)
+        rc = BG_ERR_HIP;
+    if (rc == BG_OK && hipMemcpy(dstate, &host, sizeof host, hipMemcpyHostToDevice) != hipSuccess)
+        rc = BG_ERR_HIP;
+    s->path_t.clear();
+    s->path_y.clear();
+    while (rc == BG_OK) {
+        rc = bdf_launch_path(s->method, s->rhs, s->semantics, D, dstate, dpt, dpy, cap, nullptr);
+        if (rc != BG_OK) break;
+        if (hipMemcpy(&host, dstate, sizeof host, hipMemcpyDeviceToHost) != hipSuccess) {
+            rc = BG_ERR_HIP;
+            break;
+        }
+        const size_t old = s->path_t.size();
+        s->path_t.resize(old + host.count);
+        s->path_y.resize((old + host.count) * D);
+        if (host.count &&
+            (hipMemcpy(s->path_t.data() + old, dpt, sizeof(double) * host.count,
+                       hipMemcpyDeviceToHost) != hipSuccess ||
+             hipMemcpy(s->path_y.data() + old * D, dpy, sizeof(double) * host.count * D,
+                       hipMemcpyDeviceToHost) != hipSuccess)) {
+            rc = BG_ERR_HIP;
+            break;
+        }
+        if (host.finished) break;
+    }
+    if (dstate) hipFree(dstate);
+    if (dpt) hipFree(dpt);
+    if (dpy) hipFree(dpy);
+    if (rc == BG_OK) s->failure = host.status;
+    return rc;
+}
+
 extern "C" int bg_solve(bg_solver *s) {
     if (!s) return BG_ERR_INVALID_ARGUMENT;
     // rk.rs:250-256 / bdf.rs:258-264
@@ -244,7 +297,7 @@ extern "C" int bg_solve(bg_solver *s) {
     s->solved = false;
     s->fused = false;
     s->next = 0;
-    int rc = is_rk(s->method) ? solve_rk_path(s) : BG_ERR_UNSUPPORTED;
+    int rc = is_rk(s->method) ? solve_rk_path(s) : solve_bdf_path(s);
     if (rc != BG_OK) return rc;
     s->solved = true;
     return BG_OK;
@@ -314,11 +367,12 @@ struct bg_ensemble {
     int rhs = -1;
     uint32_t n_params = 0;
     uint64_t max_attempts = 0;
+    int lu_bits = 64;
     Field tol, dt_max, dt_min, t0, t_end;
     std::vector<double> y0, params, shared;
     bool have_y0 = false, have_deriv = false, solved = false;
     DevBuf d_y0, d_params, d_shared, d_t0, d_tend, d_tol, d_dtmin, d_dtmax, d_order;
-    DevBuf d_tout, d_yout, d_status, d_counts;
+    DevBuf d_tout, d_yout, d_status, d_counts, d_work;
 };
 
 static int set_field(bg_ensemble *e, Field &f, const double *v, uint64_t count) {
@@ -448,6 +502,12 @@ extern "C" int bg_ensemble_with_max_attempts(bg_ensemble *e, uint64_t max_attemp
     return BG_OK;
 }
 
+extern "C" int bg_ensemble_with_lu_precision(bg_ensemble *e, int bits) {
+    if (!e || (bits != 32 && bits != 64)) return BG_ERR_INVALID_ARGUMENT;
+    e->lu_bits = bits;
+    return BG_OK;
+}
+
 static int upload_field(DevBuf &b, const Field &f, const double **ptr, double *scalar) {
     *scalar = f.scalar;
     *ptr = nullptr;
@@ -503,6 +563,8 @@ static int ensemble_prepare(bg_ensemble *e, bg_ensemble_desc &d, bool want_order
     if ((rc = e->d_yout.ensure(e->n * e->dim * sizeof(double))) != BG_OK) return rc;
     if ((rc = e->d_status.ensure(e->n * sizeof(int32_t))) != BG_OK) return rc;
     if ((rc = e->d_counts.ensure(e->n * sizeof(bg_counts))) != BG_OK) return rc;
+    if ((rc = e->d_work.ensure(e->n * sizeof(bg_counts))) != BG_OK) return rc;
+    d.work = (bg_counts *)e->d_work.p;
     d.t_out = (double *)e->d_tout.p;
     d.y_out = (double *)e->d_yout.p;
     d.status = (int32_t *)e->d_status.p;
@@ -516,10 +578,13 @@ extern "C" int bg_ensemble_solve(bg_ensemble *e) {
     int rc = ensemble_prepare(e, d, true);
     if (rc != BG_OK) return rc;
     e->solved = false;
-    if (!is_rk(e->method)) return BG_ERR_UNSUPPORTED;
-    bg_launch lc;
-    memset(&lc, 0, sizeof lc);
-    rc = rk_launch_ensemble(e->method, e->rhs, e->semantics, d, lc, nullptr);
+    if (is_rk(e->method)) {
+        bg_launch lc;
+        memset(&lc, 0, sizeof lc);
+        rc = rk_launch_ensemble(e->method, e->rhs, e->semantics, d, lc, nullptr);
+    } else {
+        rc = bdf_launch_ensemble(e->method, e->rhs, e->semantics, e->lu_bits == 32, d, nullptr);
+    }
     if (rc != BG_OK) return rc;
     BG_HIP_CHECK(hipStreamSynchronize(nullptr));
     e->solved = true;
@@ -558,6 +623,14 @@ extern "C" int bg_ensemble_results(bg_ensemble *e, double *t_out, double *y_out,
     return BG_OK;
 }
 
+extern "C" int bg_ensemble_work(bg_ensemble *e, bg_counts *work) {
+    if (!e || !work) return BG_ERR_INVALID_ARGUMENT;
+    if (!e->solved) return BG_ERR_STATE;
+    if (e->n == 0) return BG_OK;
+    BG_HIP_CHECK(hipMemcpy(work, e->d_work.p, e->n * sizeof(bg_counts), hipMemcpyDeviceToHost));
+    return BG_OK;
+}
+
 extern "C" void bg_ensemble_destroy(bg_ensemble *e) { delete e; }
 
 // ===========================================================================
@@ -576,6 +649,12 @@ extern "C" int bg_rk_solve_plain_device(int method, int rhs, int semantics,
                                         const bg_ensemble_desc *d, void *stream) {
     if (!d || !is_rk(method)) return BG_ERR_INVALID_ARGUMENT;
     return rk_launch_plain(method, rhs, semantics, *d, (hipStream_t)stream);
+}
+
+extern "C" int bg_bdf_solve_device(int method, int rhs, int semantics, int lu_bits,
+                                   const bg_ensemble_desc *d, void *stream) {
+    if (!d || !is_bdf(method) || (lu_bits != 32 && lu_bits != 64)) return BG_ERR_INVALID_ARGUMENT;
+    return bdf_launch_ensemble(method, rhs, semantics, lu_bits == 32, *d, (hipStream_t)stream);
 }
 
 extern "C" int bg_version(void) { return 1; }

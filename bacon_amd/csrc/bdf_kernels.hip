@@ -1,0 +1,723 @@
+// bdf_kernels.hip — batched BDF (BDF6 / BDF2) for gfx950.
+//
+// Hot path: bacon-sci's BDFSolver::step (src/ivp/bdf.rs:495-634) with its
+// RK4 bootstrap (bdf.rs:346-387), finite-difference Jacobian (bdf.rs:390-411)
+// and Broyden corrector (bdf.rs:414-475), recast as an ensemble.
+//
+// Layout: one instance per group of W = D lanes (a 32-lane half-wave for the
+// 32-component Brusselator, one lane for scalar problems).  Lane g of a group
+// owns component g of every vector (state, guess, residuals, the history of
+// the last O accepted values) and row g of the Jacobian inverse, all in VGPRs.
+// Group collectives use cross-lane shuffles (ds_bpermute); the column sums of
+// the Broyden update u = s^T J^-1 go through an LDS transpose (row stride
+// W+1 doubles: conflict-free column reads).  The inverse is Gauss-Jordan with
+// partial pivoting on the rows (implicit, no row swaps); its arithmetic type
+// is a template parameter (double, or float with the Broyden iteration and
+// all residuals in double).  Instances are refilled per group from a global
+// work counter, like the RK ensemble kernel.
+#include "bg_device.h"
+#include "bg_internal.h"
+
+namespace bg {
+
+static constexpr uint32_t kBdfDefaultMaxSteps = 1000000u;
+
+// ---------------------------------------------------------------------------
+// group collectives
+// ---------------------------------------------------------------------------
+template <int W>
+struct Grp {
+    int g;  // lane index in the group
+    __device__ __forceinline__ double bcast(double v, int src) const {
+        if constexpr (W == 1) return v;
+        return __shfl(v, src, W);
+    }
+    __device__ __forceinline__ double sum(double v) const {
+#pragma unroll
+        for (int m = W / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, W);
+        return v;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// right-hand sides evaluated cooperatively: lane g returns f_g(t, y)
+// ---------------------------------------------------------------------------
+template <template <int> class R>
+struct ScalarBdf {  // the 1-D reference-test right-hand sides
+    static constexpr int D = 1, P = R<1>::P;
+    template <class G>
+    __device__ static bool lane_eval(double t, double y, const double *p, const G &, double &f) {
+        double yy[1] = {y}, ff[1];
+        const bool ok = R<1>::eval(t, yy, p, ff);
+        f = ff[0];
+        return ok;
+    }
+};
+
+// 1-D Brusselator reaction-diffusion, CELLS cells, periodic, dx = 1/CELLS;
+// y = [u_0..u_{CELLS-1}, v_0..v_{CELLS-1}], params A, B, alpha.
+template <int CELLS>
+struct RhsBrusselatorG {
+    static constexpr int D = 2 * CELLS, P = 3;
+    template <class G>
+    __device__ static bool lane_eval(double, double y, const double *p, const G &grp, double &f) {
+        const int g = grp.g;
+        const bool isu = g < CELLS;
+        const int c = isu ? g : g - CELLS;
+        const int base = isu ? 0 : CELLS;
+        const double left = grp.bcast(y, base + (c + CELLS - 1) % CELLS);
+        const double right = grp.bcast(y, base + (c + 1) % CELLS);
+        const double other = grp.bcast(y, isu ? g + CELLS : g - CELLS);
+        const double u = isu ? y : other, v = isu ? other : y;
+        const double A = p[0], B = p[1], alpha = p[2];
+        const double dx = 1.0 / CELLS;
+        const double cl = alpha / (dx * dx);
+        const double uuv = u * u * v;
+        const double lap = left - 2.0 * y + right;
+        f = isu ? A + uuv - (B + 1.0) * u + cl * lap : B * u - uuv + cl * lap;
+        return true;
+    }
+    // FLOPs of one full evaluation (all 2*CELLS components)
+    static constexpr int FLOPS = CELLS * (6 + 4 + 4 + 5);
+};
+
+// quirk register as compile-time data (bdf.rs:403-407, 568, 622)
+template <bool AsBuilt>
+struct BdfSem {
+    static constexpr double fd_sign = AsBuilt ? 1.0 : -1.0;  // Q3
+    static constexpr bool lower_own_hist = !AsBuilt;         // Q4
+    static constexpr double theta = AsBuilt ? 0.0 : 1.0;     // Q5
+    static constexpr bool restore_mul = !AsBuilt;            // Q6
+};
+
+// BDF coefficients (bdf.rs:646-672, 713-729)
+template <int O>
+struct BdfCoef;
+template <>
+struct BdfCoef<7> {
+    __device__ static constexpr double hi(int k) {
+        constexpr double h = 147.0;
+        constexpr double c[7] = {60.0 / h, -360.0 / h, 450.0 / h, -400.0 / h,
+                                 225.0 / h, -72.0 / h, 10.0 / h};
+        return c[k];
+    }
+    __device__ static constexpr double lo(int k) {
+        constexpr double l = 137.0;
+        constexpr double c[7] = {60.0 / l, -300.0 / l, 300.0 / l, -200.0 / l,
+                                 75.0 / l, -12.0 / l, 0.0};
+        return c[k];
+    }
+};
+template <>
+struct BdfCoef<3> {
+    __device__ static constexpr double hi(int k) {
+        constexpr double c[3] = {2.0 / 3.0, -4.0 / 3.0, 1.0 / 3.0};
+        return c[k];
+    }
+    __device__ static constexpr double lo(int k) {
+        constexpr double c[3] = {1.0, -1.0, 0.0};
+        return c[k];
+    }
+};
+
+enum : int { BDF_YIELD = -1, BDF_REDO = -2, BDF_DONE = -3 };
+
+// ---------------------------------------------------------------------------
+// one group's BDF solver state (per lane: its component / its row)
+// ---------------------------------------------------------------------------
+template <class RHS, int O, class Sem, class LuT>
+struct BdfLane {
+    static constexpr int D = RHS::D, W = D;
+    using C = BdfCoef<O>;
+    Grp<W> grp;
+    double *lds;  // this group's W x (W+1) transpose buffer
+    // instance scalars (replicated across the group)
+    double t, dt, end, tol, dt_min, dt_max;
+    int yield_memory;
+    bool hist_full;
+    double hist_t[O];
+    // per-lane component data
+    double y, save;
+    double hist[O];  // prev_values front..back (own component)
+    double jinv[W];  // row g of the Jacobian inverse
+    double p[RHS::P > 0 ? RHS::P : 1];
+    // counters
+    uint32_t n_newton, n_lu;
+
+    __device__ __forceinline__ bool f(double tt, double yy, double &out) const {
+        return RHS::lane_eval(tt, yy, p, grp, out);
+    }
+
+    // hist push_back + pop_front on a full deque
+    __device__ __forceinline__ void hist_shift(double tt, double yy) {
+#pragma unroll
+        for (int k = 0; k < O - 1; ++k) {
+            hist[k] = hist[k + 1];
+            hist_t[k] = hist_t[k + 1];
+        }
+        hist[O - 1] = yy;
+        hist_t[O - 1] = tt;
+    }
+
+    // bdf.rs:346-387: classical RK4, `iterations` steps; with fill = true the
+    // deque is rebuilt from the states after steps 1..O (O == iterations).
+    template <int IT, bool FILL>
+    __device__ __forceinline__ int runge_kutta() {
+#pragma unroll
+        for (int i = 0; i < IT; ++i) {
+            double k1, k2, k3, k4;
+            if (!f(t, y, k1)) return BG_USER_ERROR;
+            k1 *= dt;
+            if (!f(t + 0.5 * dt, y + k1 * 0.5, k2)) return BG_USER_ERROR;
+            k2 *= dt;
+            if (!f(t + 0.5 * dt, y + k2 * 0.5, k3)) return BG_USER_ERROR;
+            k3 *= dt;
+            if (!f(t + dt, y + k3, k4)) return BG_USER_ERROR;
+            k4 *= dt;
+            if (FILL && i != 0) {
+                hist[i - 1] = y;
+                hist_t[i - 1] = t;
+            }
+            y += (k1 + k2 * 2.0 + k3 * 2.0 + k4) * (1.0 / 6.0);
+            t += dt;
+        }
+        if (FILL) {
+            hist[IT - 1] = y;
+            hist_t[IT - 1] = t;
+            hist_full = true;
+        }
+        return BG_OK;
+    }
+
+    // residual closures bdf.rs:548-575 (lane g's component)
+    template <bool LOWER>
+    __device__ __forceinline__ bool residual(double tt, double x, double &g) const {
+        double fx;
+        if (!f(tt, x, fx)) return false;
+        constexpr bool own = LOWER && Sem::lower_own_hist;
+        g = (-fx) * dt * (LOWER ? C::lo(0) : C::hi(0));
+#pragma unroll
+        for (int ind = 1; ind < O; ++ind)
+            g += hist[O - ind] * (own ? C::lo(ind) : C::hi(ind));
+        g = g + x;
+        return true;
+    }
+
+    // y_g = -(Jinv x)_g with x broadcast through LDS
+    __device__ __forceinline__ double neg_jinv_times(double x) {
+        lds[grp.g] = x;
+        __builtin_amdgcn_wave_barrier();
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < W; ++j) acc += jinv[j] * lds[j];
+        __builtin_amdgcn_wave_barrier();
+        return -acc;
+    }
+
+    // In-place Gauss-Jordan inversion (exchange steps) of the group matrix
+    // whose row g is staged in row g of the LDS buffer, with partial pivoting on rows that are not swapped:
+    // column k pivots on the unused row p with the largest |a(p,k)| (lowest
+    // lane on ties).  Afterwards lane p holds row col(p) of the inverse with
+    // its columns permuted by col(), which one LDS scatter/gather undoes.
+    // Arithmetic in LuT; returns false on an exact zero (or NaN) pivot.
+    __device__ __forceinline__ bool invert() {
+        LuT b[W];
+#pragma unroll
+        for (int j = 0; j < W; ++j) b[j] = (LuT)lds[grp.g * (W + 1) + j];
+        __builtin_amdgcn_wave_barrier();
+        bool used = false, ok = true;
+        int my_col = 0;
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            double best = used ? -1.0 : fabs((double)b[k]);
+            int who = grp.g;
+#pragma unroll
+            for (int m = W / 2; m >= 1; m >>= 1) {
+                const double ob = __shfl_xor(best, m, W);
+                const int ow = __shfl_xor(who, m, W);
+                if (ob > best || (ob == best && ow < who)) {
+                    best = ob;
+                    who = ow;
+                }
+            }
+            if (!(best > 0.0)) ok = false;
+            const bool piv = (who == grp.g);
+            if (piv) {
+                used = true;
+                my_col = k;
+                const LuT ip = (LuT)1 / b[k];
+#pragma unroll
+                for (int j = 0; j < W; ++j)
+                    if (j != k) b[j] *= ip;
+                b[k] = ip;
+#pragma unroll
+                for (int j = 0; j < W; ++j) lds[j] = (double)b[j];
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (!piv) {
+                const LuT f = b[k];
+#pragma unroll
+                for (int j = 0; j < W; ++j)
+                    if (j != k) b[j] = b[j] - f * (LuT)lds[j];
+                b[k] = -f * (LuT)lds[k];
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        // lane p: row col(p) of the inverse, column c belongs to inverse
+        // column q with col(q) = c.  Scatter rows by col(), publish col() in
+        // the padding column, gather with the column permutation.
+#pragma unroll
+        for (int c = 0; c < W; ++c) lds[my_col * (W + 1) + c] = (double)b[c];
+        lds[grp.g * (W + 1) + W] = (double)my_col;
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+            int cq = (int)lds[q * (W + 1) + W];
+            cq = cq < 0 ? 0 : (cq >= W ? W - 1 : cq);
+            jinv[q] = lds[grp.g * (W + 1) + cq];
+        }
+        __builtin_amdgcn_wave_barrier();
+        return ok;
+    }
+
+    // bdf.rs:414-475; on success `guess` holds the corrected value
+    template <bool LOWER>
+    __device__ __forceinline__ int secant(double &guess) {
+        const double tt = t + Sem::theta * dt;
+        guess = y;
+        double F;
+        if (!residual<LOWER>(tt, guess, F)) return BG_USER_ERROR;
+        // bdf.rs:390-411: columns (G(x + dt e_c) +/- G(x - dt e_c)) / (2 dt);
+        // lane g stores J(g, c) into row g of the group's LDS buffer
+        const double denom = 1.0 / (2.0 * dt);
+#pragma unroll 1
+        for (int c = 0; c < W; ++c) {
+            const bool mine = (grp.g == c);
+            if (mine) guess += dt;
+            double above, below;
+            if (!residual<LOWER>(tt, guess, above)) return BG_USER_ERROR;
+            if (mine) guess -= 2.0 * dt;
+            if (!residual<LOWER>(tt, guess, below)) return BG_USER_ERROR;
+            if (mine) guess += dt;
+            lds[grp.g * (W + 1) + c] = (above + Sem::fd_sign * below) * denom;
+        }
+        __builtin_amdgcn_wave_barrier();
+        ++n_lu;
+        if (!invert()) return BG_SINGULAR_MATRIX;
+        double s = neg_jinv_times(F);
+        guess += s;
+#pragma unroll 1
+        for (int n = 2; n < 1000; ++n) {
+            const double Fl = F;
+            if (!residual<LOWER>(tt, guess, F)) return BG_USER_ERROR;
+            ++n_newton;
+            const double adj = neg_jinv_times(F - Fl);
+            // p = (-s)^T adj ; u = s^T Jinv (column sums through LDS)
+            const double pp = grp.sum(adj * (-s));
+#pragma unroll
+            for (int j = 0; j < W; ++j) lds[grp.g * (W + 1) + j] = jinv[j] * s;
+            __builtin_amdgcn_wave_barrier();
+            double u = 0.0;
+#pragma unroll
+            for (int k = 0; k < W; ++k) u += lds[k * (W + 1) + grp.g];
+            __builtin_amdgcn_wave_barrier();
+            lds[grp.g] = u;
+            __builtin_amdgcn_wave_barrier();
+            const double w = (s + adj) / pp;
+#pragma unroll
+            for (int j = 0; j < W; ++j) jinv[j] += lds[j] * w;
+            __builtin_amdgcn_wave_barrier();
+            s = neg_jinv_times(F);
+            guess += s;
+            const double nrm = sqrt(grp.sum(s * s));
+            if (nrm <= tol) return BG_OK;
+        }
+        return BG_MAXIMUM_ITERATIONS_EXCEEDED;
+    }
+
+    // One call of BDFSolver::step (bdf.rs:495-634).  Returns BDF_YIELD,
+    // BDF_REDO, BDF_DONE or an IVPError code.  On a yield, (*yt, *yv) is the
+    // yielded value (own component).
+    __device__ __forceinline__ int step(double *yt, double *yv, bool *rejected) {
+        *rejected = false;
+        if (yield_memory > 0 && yield_memory <= O) {  // bdf.rs:500-512
+            const int item = O - yield_memory;
+            yield_memory -= 1;
+            if (yield_memory == 0) yield_memory = O + 2;
+            double vt = hist_t[0], vy = hist[0];
+#pragma unroll
+            for (int k = 1; k < O; ++k)
+                if (item == k) {
+                    vt = hist_t[k];
+                    vy = hist[k];
+                }
+            *yt = vt;
+            *yv = vy;
+            return BDF_YIELD;
+        }
+        if (yield_memory == O + 2) {  // bdf.rs:519-525
+            yield_memory = 0;
+            hist_shift(t, y);
+            *yt = t;
+            *yv = y;
+            return BDF_YIELD;
+        }
+        if (t >= end) return BDF_DONE;
+        if (t + dt >= end) {  // bdf.rs:531-535
+            dt = end - t;
+            const int rc = runge_kutta<1, false>();
+            if (rc) return rc;
+            *yt = t;
+            *yv = y;
+            return BDF_YIELD;
+        }
+        if (!hist_full) {  // bdf.rs:537-546
+            save = y;
+            if (t + dt * (double)O >= end) dt = (end - t) / (double)O;
+            const int rc = runge_kutta<O, true>();
+            if (rc) return rc;
+            yield_memory = O + 1;
+            return BDF_REDO;
+        }
+        double hi, lo;
+        int rc = secant<false>(hi);
+        if (rc) return rc;
+        rc = secant<true>(lo);
+        if (rc) return rc;
+        const double dlt = hi - lo;
+        const double error = sqrt(grp.sum(dlt * dlt));
+        if (error <= tol) {  // bdf.rs:583-614
+            y = hi;
+            t += dt;
+            if (yield_memory == O + 1) {
+                yield_memory -= 1;
+                return BDF_REDO;
+            }
+            hist_shift(t, y);
+            if (error < 0.1 * tol) {
+                dt *= 2.0;
+                if (dt > dt_max) dt = dt_max;
+                hist_full = false;
+            }
+            *yt = t;
+            *yv = y;
+            return BDF_YIELD;
+        }
+        *rejected = true;
+        if (yield_memory == O + 1) {  // bdf.rs:620-624
+            if (Sem::restore_mul)
+                t -= dt * (double)O;
+            else
+                t -= dt - (double)O;  // Q6
+            y = save;
+        }
+        dt *= 0.5;
+        if (dt < dt_min) return BG_MINIMUM_TIME_DELTA_EXCEEDED;
+        hist_full = false;
+        return BDF_REDO;
+    }
+
+    __device__ __forceinline__ void load(const bg_ensemble_desc &d, uint64_t i) {
+        const uint64_t n = d.n;
+        y = d.y0[(uint64_t)grp.g * n + i];
+#pragma unroll
+        for (int k = 0; k < RHS::P; ++k) p[k] = d.params[(uint64_t)k * n + i];
+        t = d.t0 ? d.t0[i] : d.t0_s;
+        end = d.t_end ? d.t_end[i] : d.t_end_s;
+        tol = d.tol ? d.tol[i] : d.tol_s;
+        dt_min = d.dt_min ? d.dt_min[i] : d.dt_min_s;
+        dt_max = d.dt_max ? d.dt_max[i] : d.dt_max_s;
+        dt = (dt_max + dt_min) * 0.5;  // bdf.rs:302
+        yield_memory = 0;
+        hist_full = false;
+        save = y;
+        n_newton = n_lu = 0;
+#pragma unroll
+        for (int k = 0; k < O; ++k) {
+            hist[k] = 0.0;
+            hist_t[k] = 0.0;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// persistent ensemble kernel: one instance per group, group refill
+// ---------------------------------------------------------------------------
+template <class RHS, int O, class Sem, class LuT>
+__global__ void __launch_bounds__(64)
+    bdf_ensemble_kernel(const bg_ensemble_desc *__restrict__ dp, uint32_t *__restrict__ counter) {
+    constexpr int W = RHS::D;
+    constexpr int GPW = 64 / W;  // groups per wave
+    __shared__ double lds_all[GPW][W * (W + 1)];
+    const bg_ensemble_desc d = *dp;
+    const int lane = __lane_id();
+    const int grp_id = lane / W;
+    BdfLane<RHS, O, Sem, LuT> L;
+    L.grp.g = lane % W;
+    L.lds = lds_all[grp_id];
+    const uint32_t cap =
+        (uint32_t)((d.max_attempts && d.max_attempts < kBdfDefaultMaxSteps) ? d.max_attempts
+                                                                          : kBdfDefaultMaxSteps);
+    int64_t inst = -1;  // -1 need work, -2 exhausted
+    for (;;) {
+        // group leaders that need work
+        const bool leader = (L.grp.g == 0);
+        const uint64_t need = __ballot(leader && inst == -1);
+        if (need != 0) {
+            const int first = __ffsll((unsigned long long)need) - 1;
+            uint32_t base = 0;
+            if (lane == first) base = atomicAdd(counter, (uint32_t)__popcll(need));
+            base = __shfl(base, first);
+            // rank of my group's leader among needing leaders
+            const uint64_t below = need & ((1ull << (grp_id * W)) - 1ull);
+            const bool my_need = (need >> (grp_id * W)) & 1ull;
+            if (my_need) {
+                const uint64_t k = (uint64_t)base + (uint64_t)__popcll(below);
+                if (k < d.n) {
+                    inst = d.order ? (int64_t)d.order[k] : (int64_t)k;
+                    L.load(d, (uint64_t)inst);
+                } else {
+                    inst = -2;
+                }
+            }
+        }
+        if (__ballot(inst >= 0) == 0) break;
+        if (inst >= 0) {
+            uint32_t steps = 0, nacc = 0, nrej = 0;
+            int st;
+            for (;;) {
+                if (steps >= cap) {
+                    st = BG_MAXIMUM_ITERATIONS_EXCEEDED;
+                    break;
+                }
+                ++steps;
+                double yt, yv;
+                bool rej;
+                const int r = L.step(&yt, &yv, &rej);
+                if (rej) ++nrej;
+                if (r == BDF_YIELD) {
+                    ++nacc;
+                } else if (r == BDF_DONE) {
+                    st = BG_OK;
+                    break;
+                } else if (r != BDF_REDO) {
+                    st = r;
+                    break;
+                }
+            }
+            const uint64_t n = d.n, i = (uint64_t)inst;
+            if (d.y_out) d.y_out[(uint64_t)L.grp.g * n + i] = L.y;
+            if (L.grp.g == 0) {
+                if (d.t_out) d.t_out[i] = L.t;
+                if (d.status) d.status[i] = st;
+                if (d.counts) {
+                    bg_counts c;
+                    c.n_accepted = nacc;
+                    c.n_rejected = nrej;
+                    d.counts[i] = c;
+                }
+                if (d.work) {
+                    bg_counts c;
+                    c.n_accepted = L.n_newton;
+                    c.n_rejected = L.n_lu;
+                    d.work[i] = c;
+                }
+            }
+            inst = -1;
+        }
+    }
+}
+
+// Single-instance path production (one group): every yield is written into
+// the path buffer; resumable when the buffer is full.
+template <class RHS, int O, class Sem, class LuT>
+__global__ void __launch_bounds__(64)
+    bdf_path_kernel(BdfPathState *__restrict__ st, double *__restrict__ path_t,
+                    double *__restrict__ path_y, uint32_t cap) {
+    constexpr int W = RHS::D;
+    __shared__ double lds[W * (W + 1)];
+    const int lane = __lane_id();
+    if (lane >= W) return;
+    BdfLane<RHS, O, Sem, LuT> L;
+    L.grp.g = lane;
+    L.lds = lds;
+    L.t = st->t;
+    L.dt = st->dt;
+    L.end = st->end;
+    L.tol = st->tol;
+    L.dt_min = st->dt_min;
+    L.dt_max = st->dt_max;
+    L.yield_memory = st->yield_memory;
+    L.hist_full = st->hist_full != 0;
+    L.y = st->y[lane];
+    L.save = st->save[lane];
+#pragma unroll
+    for (int k = 0; k < O; ++k) {
+        L.hist[k] = st->hist[k][lane];
+        L.hist_t[k] = st->hist_t[k];
+    }
+#pragma unroll
+    for (int k = 0; k < RHS::P; ++k) L.p[k] = st->p[k];
+    L.n_newton = st->n_newton;
+    L.n_lu = st->n_lu;
+    uint32_t count = 0;
+    int status = BG_OK, finished = st->finished;
+    uint64_t steps = st->steps;
+    uint32_t nacc = st->n_accepted, nrej = st->n_rejected;
+    while (!finished && count < cap) {
+        if (steps >= st->max_steps) {
+            status = BG_MAXIMUM_ITERATIONS_EXCEEDED;
+            finished = 1;
+            break;
+        }
+        ++steps;
+        double yt, yv;
+        bool rej;
+        const int r = L.step(&yt, &yv, &rej);
+        if (rej) ++nrej;
+        if (r == BDF_YIELD) {
+            ++nacc;
+            if (lane == 0) path_t[count] = yt;
+            path_y[(uint64_t)count * W + lane] = yv;
+            ++count;
+        } else if (r == BDF_DONE) {
+            finished = 1;
+        } else if (r != BDF_REDO) {
+            status = r;
+            finished = 1;
+        }
+    }
+    st->y[lane] = L.y;
+    st->save[lane] = L.save;
+#pragma unroll
+    for (int k = 0; k < O; ++k) st->hist[k][lane] = L.hist[k];
+    if (lane == 0) {
+        st->t = L.t;
+        st->dt = L.dt;
+        st->yield_memory = L.yield_memory;
+        st->hist_full = L.hist_full ? 1 : 0;
+#pragma unroll
+        for (int k = 0; k < O; ++k) st->hist_t[k] = L.hist_t[k];
+        st->steps = steps;
+        st->n_accepted = nacc;
+        st->n_rejected = nrej;
+        st->n_newton = L.n_newton;
+        st->n_lu = L.n_lu;
+        st->status = status;
+        st->finished = finished;
+        st->count = count;
+    }
+}
+
+__global__ void bdf_setup_kernel(uint32_t *counter, bg_ensemble_desc *dst, const bg_ensemble_desc d) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *counter = 0u;
+        *dst = d;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// dispatch
+// ---------------------------------------------------------------------------
+template <class T>
+struct BTag {
+    using type = T;
+};
+
+template <class V>
+static int visit_bdf(int method, int rhs, int semantics, uint32_t dim, int lu_fp32, V &&v) {
+    if (semantics != BG_AS_BUILT && semantics != BG_INTENDED) return BG_ERR_INVALID_ARGUMENT;
+    auto with_order = [&](auto rtag, auto ftag) -> int {
+        auto with_sem = [&](auto otag) -> int {
+            return semantics == BG_AS_BUILT ? v(rtag, otag, BTag<BdfSem<true>>{}, ftag)
+                                            : v(rtag, otag, BTag<BdfSem<false>>{}, ftag);
+        };
+        if (method == BG_BDF6) return with_sem(std::integral_constant<int, 7>{});
+        if (method == BG_BDF2) return with_sem(std::integral_constant<int, 3>{});
+        return BG_ERR_UNSUPPORTED;
+    };
+    switch (rhs) {
+    case BG_RHS_EXP:
+        return dim == 1 ? with_order(BTag<ScalarBdf<RhsExp>>{}, BTag<double>{}) : BG_ERR_UNSUPPORTED;
+    case BG_RHS_NEG:
+        return dim == 1 ? with_order(BTag<ScalarBdf<RhsNeg>>{}, BTag<double>{}) : BG_ERR_UNSUPPORTED;
+    case BG_RHS_QUADRATIC:
+        return dim == 1 ? with_order(BTag<ScalarBdf<RhsQuadratic>>{}, BTag<double>{})
+                        : BG_ERR_UNSUPPORTED;
+    case BG_RHS_COS:
+        return dim == 1 ? with_order(BTag<ScalarBdf<RhsCos>>{}, BTag<double>{}) : BG_ERR_UNSUPPORTED;
+    case BG_RHS_FAIL_AFTER:
+        return dim == 1 ? with_order(BTag<ScalarBdf<RhsFailAfter>>{}, BTag<double>{})
+                        : BG_ERR_UNSUPPORTED;
+    case BG_RHS_BRUSSELATOR: {
+        if (dim != 32 || method != BG_BDF6) return BG_ERR_UNSUPPORTED;
+        auto o7 = std::integral_constant<int, 7>{};
+        auto rt = BTag<RhsBrusselatorG<16>>{};
+        if (lu_fp32)
+            return semantics == BG_AS_BUILT ? v(rt, o7, BTag<BdfSem<true>>{}, BTag<float>{})
+                                            : v(rt, o7, BTag<BdfSem<false>>{}, BTag<float>{});
+        return semantics == BG_AS_BUILT ? v(rt, o7, BTag<BdfSem<true>>{}, BTag<double>{})
+                                        : v(rt, o7, BTag<BdfSem<false>>{}, BTag<double>{});
+    }
+    default:
+        return BG_ERR_UNSUPPORTED;
+    }
+}
+
+bool bdf_supported(int method, int rhs, uint32_t dim) {
+    return visit_bdf(method, rhs, BG_INTENDED, dim, 0, [](auto, auto, auto, auto) { return BG_OK; }) ==
+           BG_OK;
+}
+
+int bdf_launch_ensemble(int method, int rhs, int semantics, int lu_fp32, const bg_ensemble_desc &d,
+                        hipStream_t stream) {
+    return visit_bdf(method, rhs, semantics, d.dim, lu_fp32,
+                     [&](auto rtag, auto otag, auto stag, auto ftag) -> int {
+                         using R = typename decltype(rtag)::type;
+                         using S = typename decltype(stag)::type;
+                         using F = typename decltype(ftag)::type;
+                         constexpr int O = decltype(otag)::value;
+                         if (d.n == 0) return BG_OK;
+                         if (d.n > 0x7fffffffull || !d.y0) return BG_ERR_INVALID_ARGUMENT;
+                         if (R::P > 0 && (!d.params || d.n_params < (uint32_t)R::P))
+                             return BG_ERR_INVALID_ARGUMENT;
+                         constexpr int W = R::D;
+                         constexpr int GPW = 64 / W;
+                         int dev = 0;
+                         BG_HIP_CHECK(hipGetDevice(&dev));
+                         int per_cu = 0;
+                         BG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                             &per_cu, bdf_ensemble_kernel<R, O, S, F>, 64, 0));
+                         if (per_cu < 1) per_cu = 1;
+                         uint64_t grid = (uint64_t)per_cu * (uint64_t)device_cu_count(dev);
+                         const uint64_t need = (d.n + GPW - 1) / GPW;
+                         if (grid > need) grid = need;
+                         char *ws = nullptr;
+                         BG_HIP_CHECK(
+                             hipMallocAsync((void **)&ws, 256 + sizeof(bg_ensemble_desc), stream));
+                         uint32_t *counter = (uint32_t *)ws;
+                         bg_ensemble_desc *ddesc = (bg_ensemble_desc *)(ws + 256);
+                         bdf_setup_kernel<<<1, 64, 0, stream>>>(counter, ddesc, d);
+                         bdf_ensemble_kernel<R, O, S, F>
+                             <<<(uint32_t)grid, 64, 0, stream>>>(ddesc, counter);
+                         BG_HIP_CHECK(hipGetLastError());
+                         BG_HIP_CHECK(hipFreeAsync(ws, stream));
+                         return BG_OK;
+                     });
+}
+
+int bdf_launch_path(int method, int rhs, int semantics, uint32_t dim, BdfPathState *state,
+                    double *path_t, double *path_y, uint32_t cap, hipStream_t stream) {
+    return visit_bdf(method, rhs, semantics, dim, 0,
+                     [&](auto rtag, auto otag, auto stag, auto ftag) -> int {
+                         using R = typename decltype(rtag)::type;
+                         using S = typename decltype(stag)::type;
+                         using F = typename decltype(ftag)::type;
+                         constexpr int O = decltype(otag)::value;
+                         bdf_path_kernel<R, O, S, F><<<1, 64, 0, stream>>>(state, path_t, path_y, cap);
+                         BG_HIP_CHECK(hipGetLastError());
+                         return BG_OK;
+                     });
+}
+
+}  // namespace bg

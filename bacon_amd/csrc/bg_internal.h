@@ -34,6 +34,24 @@ struct PathState {
 int rk_launch_path(int method, int rhs, int semantics, uint32_t dim, PathState *state,
                    double *path_t, double *path_y, uint32_t cap, hipStream_t stream);
 
+// BDF family (bdf_kernels.hip)
+struct BdfPathState {
+    double t, dt, end, tol, dt_min, dt_max;
+    uint64_t steps, max_steps;
+    uint32_t n_accepted, n_rejected, n_newton, n_lu;
+    int32_t yield_memory, hist_full, status, finished;
+    uint32_t count, pad;
+    double y[64], save[64];
+    double hist[7][64];
+    double hist_t[7];
+    double p[8];
+};
+bool bdf_supported(int method, int rhs, uint32_t dim);
+int bdf_launch_ensemble(int method, int rhs, int semantics, int lu_fp32, const bg_ensemble_desc &d,
+                        hipStream_t stream);
+int bdf_launch_path(int method, int rhs, int semantics, uint32_t dim, BdfPathState *state,
+                    double *path_t, double *path_y, uint32_t cap, hipStream_t stream);
+
 int device_cu_count(int device);
 
 }  // namespace bg

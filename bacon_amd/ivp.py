@@ -269,6 +269,12 @@ class Ensemble:
         _check(L.lib().bg_ensemble_with_max_attempts(self._h, n))
         return self
 
+    def with_lu_precision(self, bits: int):
+        """BDF: arithmetic of the Jacobian inverse (64, or 32 with FP64
+        residuals and Broyden iteration)."""
+        _check(L.lib().bg_ensemble_with_lu_precision(self._h, bits))
+        return self
+
     def solve(self, plain: bool = False):
         fn = L.lib().bg_ensemble_solve_plain if plain else L.lib().bg_ensemble_solve
         _check(fn(self._h))
@@ -282,11 +288,18 @@ class Ensemble:
         _check(L.lib().bg_ensemble_results(self._h, _dptr(t), _dptr(y),
                                            st.ctypes.data_as(C.POINTER(C.c_int32)), cnt))
         c = np.ctypeslib.as_array(cnt).view(np.uint32).reshape(-1, 2)[: self.n]
-        return dict(t=t, y=y, status=st, n_accepted=c[:, 0].astype(np.int64),
-                    n_rejected=c[:, 1].astype(np.int64))
+        out = dict(t=t, y=y, status=st, n_accepted=c[:, 0].astype(np.int64),
+                   n_rejected=c[:, 1].astype(np.int64))
+        if self.method in (L.BDF6, L.BDF2):
+            wk = (L.Counts * max(self.n, 1))()
+            _check(L.lib().bg_ensemble_work(self._h, wk))
+            w = np.ctypeslib.as_array(wk).view(np.uint32).reshape(-1, 2)[: self.n]
+            out.update(n_newton=w[:, 0].astype(np.int64), n_lu=w[:, 1].astype(np.int64))
+        return out
 
 
-def solve_ensemble(method, w: dict, semantics=L.INTENDED, plain=False, max_attempts=0):
+def solve_ensemble(method, w: dict, semantics=L.INTENDED, plain=False, max_attempts=0,
+                   lu_bits=64):
     """Solve a workload dict (see bacon_amd.workloads) through the batched
     builder; returns final t, y, status and counters."""
     n = w["y0"].shape[1]
@@ -300,4 +313,6 @@ def solve_ensemble(method, w: dict, semantics=L.INTENDED, plain=False, max_attem
          .with_semantics(semantics))
     if max_attempts:
         e.with_max_attempts(max_attempts)
+    if lu_bits != 64:
+        e.with_lu_precision(lu_bits)
     return e.solve(plain=plain)

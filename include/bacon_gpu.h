@@ -151,6 +151,8 @@ int bg_ensemble_with_semantics(bg_ensemble *e, int semantics);
 /* cap on step attempts per instance (guards the reference's endless Redo on
  * a NaN error); 0 = default */
 int bg_ensemble_with_max_attempts(bg_ensemble *e, uint64_t max_attempts);
+/* BDF: arithmetic of the Jacobian inverse, 64 (default) or 32 bits */
+int bg_ensemble_with_lu_precision(bg_ensemble *e, int bits);
 /* Solve on the GPU (synchronous). */
 int bg_ensemble_solve(bg_ensemble *e);
 /* Plain path: the same device step code launched once per instance,
@@ -159,6 +161,8 @@ int bg_ensemble_solve_plain(bg_ensemble *e);
 /* Copy results to host arrays (any may be NULL); y_out is [dim][n]. */
 int bg_ensemble_results(bg_ensemble *e, double *t_out, double *y_out, int32_t *status,
                         bg_counts *counts);
+/* BDF work counters per instance: (Broyden iterations, Jacobian inverses) */
+int bg_ensemble_work(bg_ensemble *e, bg_counts *work);
 void bg_ensemble_destroy(bg_ensemble *e);
 
 /* ======================================================================== */
@@ -190,6 +194,7 @@ typedef struct {
     double *y_out;         /* [dim][n]        */
     int32_t *status;       /* [n]             */
     bg_counts *counts;     /* [n]             */
+    bg_counts *work;       /* [n] BDF only: (Broyden iterations, Jacobian inverses) */
 } bg_ensemble_desc;
 
 /* Batched RK (RK45 / RK23) in one persistent launch on `stream`
@@ -199,6 +204,11 @@ int bg_rk_solve_device(int method, int rhs, int semantics, const bg_ensemble_des
 /* The plain per-instance path over the same descriptor (synchronous). */
 int bg_rk_solve_plain_device(int method, int rhs, int semantics, const bg_ensemble_desc *d,
                              void *stream);
+/* Batched BDF (BDF6 / BDF2) in one persistent launch on `stream`
+ * (asynchronous).  lu_bits = 64: Jacobian inverse in FP64; 32: inverse
+ * factored in FP32, every residual and the Broyden iteration in FP64. */
+int bg_bdf_solve_device(int method, int rhs, int semantics, int lu_bits,
+                        const bg_ensemble_desc *d, void *stream);
 
 /* Library / device information */
 int bg_version(void);

@@ -812,7 +812,7 @@ typedef struct {
     int lower_own_hist;  /* 0 as built (Q4), 1 intended               */
     double f_time_theta; /* 0 as built (f at t_n, Q5), 1 intended      */
     int restore_mul;     /* 0 as built (t -= dt - O, Q6), 1: t -= O*dt */
-    int64_t n_rhs, n_newton, n_lu;
+    int64_t n_rhs, n_newton, n_lu, n_reject;
     double jinv[MAXD * MAXD], jac[MAXD * MAXD];
 } bdf_solver;
 
@@ -1052,6 +1052,7 @@ static int bdf_step(bdf_solver *s, double *yt, double *yv) {
         memcpy(yv, s->state, sizeof(double) * (size_t)D);
         return OR_STEP_YIELD;
     }
+    s->n_reject++;
     if (s->yield_memory == O + 1) { /* bdf.rs:620-624 */
         if (s->restore_mul)
             s->time -= s->dt * s->order;
@@ -1091,7 +1092,7 @@ static int bdf_solve(const or_problem *p, double *y_out, or_result *r, or_path *
             r->n_accepted++;
             path_push(path, s->D, yt, yv, 0.0);
         } else if (st == OR_STEP_REDO) {
-            /* counted below as rejections only when a BDF attempt failed */
+            /* rejected BDF attempts are counted in bdf_step */
         } else {
             status = st;
             break;
@@ -1103,6 +1104,7 @@ static int bdf_solve(const or_problem *p, double *y_out, or_result *r, or_path *
     r->n_rhs = s->n_rhs;
     r->n_newton = s->n_newton;
     r->n_lu = s->n_lu;
+    r->n_rejected = s->n_reject;
     memcpy(y_out, s->state, sizeof(double) * (size_t)s->D);
     free(s);
     return status;

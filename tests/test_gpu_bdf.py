@@ -1,0 +1,121 @@
+"""GPU parity of the batched BDF kernels (through the C ABI) against the oracle.
+
+Bars:
+  * the reference's own BDF tests (src/ivp/bdf.rs:785-1063) through the
+    single-instance builder and iterator: as built, the path is empty (the
+    first BDF step is rejected and the time restore of bdf.rs:622 ends the
+    solve); intended, every yielded value passes the test's own check and the
+    last value agrees with the oracle's within 10x the tolerance;
+  * the 32-component Brusselator ensemble: final states within 10x the
+    requested tolerance (1e-7) of the oracle for every instance, with the
+    Jacobian inverse in FP64 and in FP32; as built, every instance fails with
+    MaximumIterationsExceeded at t = 0.35 like the oracle.
+The BDF kernels reduce across lanes in a different order from the oracle, so
+the comparison is to the tolerance, not to the bit.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from bacon_amd import _lib as L
+from bacon_amd import ivp
+from bacon_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+BDF_CASES = [
+    ("exp", "exp", 1.0, (7.0, 7.0), np.exp),
+    ("unstable", "neg", 1.0, (10.0, 10.0), lambda t: np.exp(-t)),
+    ("quadratic", "quadratic", 1.0, (2.0, 1.0), lambda t: 1.0 - t * t),
+    ("sin", "cos", 0.0, (6.0, 6.0), np.sin),
+]
+
+
+def _solver(cls, rhs, y0, t_end, semantics):
+    return (cls.new(1).with_minimum_dt(1e-5).with_maximum_dt(0.1).with_tolerance(0.00001)
+            .with_initial_time(0.0).with_ending_time(t_end)
+            .with_initial_conditions_slice([y0]).with_derivative(rhs)
+            .with_semantics(semantics))
+
+
+@pytest.mark.parametrize("case", BDF_CASES, ids=[c[0] for c in BDF_CASES])
+@pytest.mark.parametrize("cls", [ivp.BDF6, ivp.BDF2])
+def test_reference_bdf_as_built_empty_path(gpu, orc, cls, case):
+    name, rhs, y0, tends, exact = case
+    t_end = tends[0] if cls is ivp.BDF6 else tends[1]
+    path = _solver(cls, rhs, y0, t_end, L.AS_BUILT).solve().collect_vec()
+    assert path == []
+
+
+@pytest.mark.parametrize("case", BDF_CASES, ids=[c[0] for c in BDF_CASES])
+@pytest.mark.parametrize("cls", [ivp.BDF6, ivp.BDF2])
+def test_reference_bdf_intended(gpu, orc, cls, case):
+    name, rhs, y0, tends, exact = case
+    t_end = tends[0] if cls is ivp.BDF6 else tends[1]
+    path = _solver(cls, rhs, y0, t_end, L.INTENDED).solve().collect_vec()
+    assert len(path) > 0
+    for t, y in path:
+        assert abs(y[0] - exact(t)) <= 0.01
+    method = O.BDF6 if cls is ivp.BDF6 else O.BDF2
+    r = O.solve(method, rhs, [y0], 0.0, t_end, 1e-5, 1e-5, 0.1, semantics=O.INTENDED,
+                path_cap=100000)
+    assert r["path_len"] > 0
+    tg, yg = path[-1]
+    to, yo = r["path_t"][-1], r["path_y"][-1]
+    assert abs(tg - to) <= 1e-9 * max(1.0, abs(to))
+    assert abs(yg[0] - yo[0]) <= 10 * 1e-5 * max(1.0, abs(yo[0]))
+
+
+def test_bdf_user_error(gpu, orc):
+    s = (ivp.BDF6.new(1).with_minimum_dt(1e-6).with_maximum_dt(0.05).with_tolerance(1e-6)
+         .with_initial_time(0.0).with_ending_time(2.0).with_initial_conditions([1.0])
+         .with_derivative("fail_after", [0.5]))
+    with pytest.raises(ivp.IVPError) as e:
+        s.solve().collect_vec()
+    assert e.value.name == "UserError"
+
+
+def _bruss(n):
+    w = W.brusselator(n)
+    w["dt_min"] = 1e-12
+    w["dt_max"] = 0.1
+    return w
+
+
+@pytest.fixture(scope="module")
+def bruss_oracle():
+    w = _bruss(64)
+    o = O.solve_ensemble(O.BDF6, "brusselator", w["y0"], w["params"], w["t_end"], w["tol"],
+                         dt_min=w["dt_min"], dt_max=w["dt_max"], semantics=O.INTENDED)
+    return w, o
+
+
+@pytest.mark.parametrize("lu_bits", [64, 32])
+def test_brusselator_ensemble_vs_oracle(gpu, orc, bruss_oracle, lu_bits):
+    w, o = bruss_oracle
+    g = ivp.solve_ensemble(L.BDF6, w, lu_bits=lu_bits)
+    assert np.all(g["status"] == 0)
+    assert np.all(np.abs(g["t"] - 10.0) < 1e-9)
+    err = np.abs(g["y"] - o["y"]).max()
+    assert err < 10 * 1e-7, err
+    # accepted-step counts follow the oracle closely
+    assert np.abs(g["n_accepted"] - o["n_accepted"]).max() <= 0.05 * o["n_accepted"].max()
+    assert np.all(g["n_lu"] > 0) and np.all(g["n_newton"] > 0)
+
+
+def test_brusselator_vs_radau(gpu, orc):
+    gold = np.load("tests/golden/bruss_golden.npz")
+    w = _bruss(8)
+    g = ivp.solve_ensemble(L.BDF6, w)
+    assert np.array_equal(w["y0"], gold["y0"])
+    assert np.abs(g["y"].T - gold["y"][:, 1]).max() < 1e-4
+
+
+def test_brusselator_as_built_fails_like_oracle(gpu, orc):
+    w = _bruss(16)
+    g = ivp.solve_ensemble(L.BDF6, w, semantics=L.AS_BUILT)
+    o = O.solve_ensemble(O.BDF6, "brusselator", w["y0"], w["params"], w["t_end"], w["tol"],
+                         dt_min=w["dt_min"], dt_max=w["dt_max"], semantics=O.AS_BUILT)
+    assert np.all(o["status"] == 9)
+    assert np.array_equal(g["status"], o["status"])
+    assert np.allclose(g["t"], o["t"])
