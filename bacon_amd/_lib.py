@@ -53,6 +53,15 @@ class EnsembleDesc(C.Structure):
     ]
 
 
+class AdjointDesc(C.Structure):
+    _fields_ = [
+        ("base", EnsembleDesc), ("n_obs", C.c_uint32), ("checkpoint_every", C.c_uint32),
+        ("max_checkpoints", C.c_uint32), ("pad", C.c_uint32),
+        ("obs_t", C.c_void_p), ("targets", C.c_void_p), ("loss", C.c_void_p),
+        ("grad_params", C.c_void_p), ("grad_y0", C.c_void_p), ("obs_y", C.c_void_p),
+    ]
+
+
 # (name, restype, argtypes) of every exported entry point
 SIGNATURES = [
     ("bg_new", C.c_int, [C.c_int, C.c_int, C.c_uint32, C.POINTER(C.c_void_p)]),
@@ -98,6 +107,8 @@ SIGNATURES = [
      [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.c_void_p]),
     ("bg_bdf_solve_device", C.c_int,
      [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.c_void_p]),
+    ("bg_rk_vjp_device", C.c_int,
+     [C.c_int, C.c_int, C.POINTER(AdjointDesc), C.c_void_p]),
     ("bg_version", C.c_int, []),
     ("bg_device_info", C.c_int,
      [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),

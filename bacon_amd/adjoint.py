@@ -1,0 +1,75 @@
+"""Discrete adjoint of the batched RK45 solve (device entry bg_rk_vjp_device).
+
+`solve_vjp` returns, for every instance, the loss
+    L = sum_k |y(t_k) - target_k|^2
+over shared observation times t_k, its gradient with respect to the
+per-instance parameters and initial state, and the states at the observation
+times.  `RK45Loss` wraps it in a torch.autograd.Function (used for gradient
+checks): its forward runs the forward and backward GPU passes, its backward
+scales the stored gradients by the incoming cotangent.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib as L
+
+
+def _ptr(t):
+    return None if t is None else t.data_ptr()
+
+
+def solve_vjp(rhs: str, y0: torch.Tensor, params: torch.Tensor, obs_t: torch.Tensor,
+              targets: torch.Tensor, t_end, tol, dt_min: float = 1e-12, dt_max: float = 0.1,
+              t0: float = 0.0, checkpoint_every: int = 16, max_checkpoints: int = 128,
+              stream=None):
+    """All tensors float64 on the GPU: y0 [D, n], params [P, n], obs_t [n_obs],
+    targets [n_obs, D, n]; t_end / tol scalars or [n] tensors."""
+    dev = y0.device
+    D, n = y0.shape
+    P = params.shape[0]
+    f64 = dict(dtype=torch.float64, device=dev)
+    y0, params = y0.contiguous(), params.contiguous()
+    obs_t, targets = obs_t.contiguous(), targets.contiguous()
+    te = t_end if torch.is_tensor(t_end) else None
+    tl = tol if torch.is_tensor(tol) else None
+    out = dict(
+        loss=torch.zeros(n, **f64), grad_params=torch.zeros((P, n), **f64),
+        grad_y0=torch.zeros((D, n), **f64), obs_y=torch.zeros((obs_t.numel(), D, n), **f64),
+        t=torch.zeros(n, **f64), y=torch.zeros((D, n), **f64),
+        status=torch.zeros(n, dtype=torch.int32, device=dev),
+        counts=torch.zeros((n, 2), dtype=torch.int32, device=dev))
+    base = L.EnsembleDesc(
+        n=n, dim=D, n_params=P, y0=_ptr(y0), params=_ptr(params),
+        t0_s=t0, t_end=_ptr(te), t_end_s=0.0 if te is not None else float(t_end),
+        tol=_ptr(tl), tol_s=0.0 if tl is not None else float(tol),
+        dt_min_s=dt_min, dt_max_s=dt_max, t_out=_ptr(out["t"]), y_out=_ptr(out["y"]),
+        status=_ptr(out["status"]), counts=_ptr(out["counts"]))
+    d = L.AdjointDesc(base=base, n_obs=obs_t.numel(), checkpoint_every=checkpoint_every,
+                      max_checkpoints=max_checkpoints, obs_t=_ptr(obs_t),
+                      targets=_ptr(targets), loss=_ptr(out["loss"]),
+                      grad_params=_ptr(out["grad_params"]), grad_y0=_ptr(out["grad_y0"]),
+                      obs_y=_ptr(out["obs_y"]))
+    sh = C.c_void_p(stream.cuda_stream if stream is not None
+                    else torch.cuda.current_stream(dev).cuda_stream)
+    rc = L.lib().bg_rk_vjp_device(L.RK45, L.RHS[rhs], C.byref(d), sh)
+    if rc != 0:
+        raise RuntimeError(f"bg_rk_vjp_device returned {rc}")
+    return out
+
+
+class RK45Loss(torch.autograd.Function):
+    """loss[i] = sum_k |y_i(t_k) - target_ik|^2 with gradients w.r.t. params and y0."""
+
+    @staticmethod
+    def forward(ctx, params, y0, rhs, obs_t, targets, t_end, tol):
+        r = solve_vjp(rhs, y0.detach(), params.detach(), obs_t, targets, t_end, tol)
+        ctx.save_for_backward(r["grad_params"], r["grad_y0"])
+        return r["loss"]
+
+    @staticmethod
+    def backward(ctx, g):
+        gp, gy = ctx.saved_tensors
+        return gp * g[None, :], gy * g[None, :], None, None, None, None, None

@@ -1,0 +1,445 @@
+// adjoint_kernels.hip — discrete adjoint of the batched RK45 step sequence.
+//
+// Extension without a counterpart in bacon-sci (its ivp module has no
+// sensitivity API).  For each instance the loss is
+//     L = sum_k |y(t_k) - target_k|^2
+// over observation times t_k.  The step controller is not differentiated: the
+// gradient is that of the frozen accepted-step map y_{n+1} = Phi(y_n; dt_n, p)
+// (bacon's RK step, src/ivp/rk.rs:370-398, with the intended tableau).
+//
+// Two persistent kernels, one lane per instance, refilled from work counters
+// like the ensemble kernel:
+//  * forward: the adaptive solve, with the end clamp of rk.rs:366-368 aimed at
+//    the next observation time (the state lands exactly on it).  Every K
+//    accepted steps it writes a checkpoint (t, dt, next stop, y) to HBM; at
+//    each observation it stores the residual y(t_k) - target_k.
+//  * backward: segments newest first.  A segment re-runs the adaptive solve
+//    from its checkpoint (bit-identical to the forward: same operations),
+//    recording its K accepted steps (t, dt, stop, y) in a per-lane buffer,
+//    then sweeps them in reverse: lambda += 2 r_k at an observation, and the
+//    stage-by-stage VJP of the step map with the right-hand side's
+//    hand-written Jacobian-vector products.
+// Storage layouts are structure-of-arrays with the instance (or lane slot) as
+// the fastest index, so each lane's loads and stores coalesce with its wave.
+#include "bg_device.h"
+#include "bg_internal.h"
+
+namespace bg {
+
+static constexpr uint32_t kAdjMaxAttempts = 10000000u;
+
+struct AdjWs {
+    double *ck;      // [maxck][F][n], F = 3 + D: t, dt, next_stop, y[D]
+    uint32_t *nck;   // [n] checkpoints written
+    double *res;     // [n_obs][D][n] residuals y(t_k) - target_k
+    double *seg;     // [K][F][slots] backward segment records
+    uint64_t slots;
+    uint32_t *counters;  // [0] forward queue, [1] backward queue
+};
+
+// Number of observation times strictly before `end` (they become stops).
+__device__ __forceinline__ uint32_t count_stops(const double *obs, uint32_t n_obs, double end) {
+    uint32_t k = 0;
+    while (k < n_obs && obs[k] < end) ++k;
+    return k;
+}
+
+template <class RHS, class Tab>
+__global__ void __launch_bounds__(256)
+    rk_adjoint_forward_kernel(const bg_adjoint_desc *__restrict__ ad, const AdjWs ws) {
+    constexpr int D = RHS::D, F = 3 + D;
+    const bg_ensemble_desc &d = ad->base;
+    const int lane = __lane_id();
+    const uint64_t n = d.n;
+    const uint32_t K = ad->checkpoint_every ? ad->checkpoint_every : 16u;
+    const uint32_t maxck = ad->max_checkpoints ? ad->max_checkpoints : 128u;
+    const uint32_t n_obs = ad->n_obs;
+    const double *obs = ad->obs_t;
+    RkLane<RHS, Tab> L;
+    int64_t inst = -1;
+    uint32_t attempts = 0, nacc = 0, nrej = 0, next_stop = 0, n_stops = 0, nck = 0;
+    double loss = 0.0;
+    for (;;) {
+        const uint64_t need = __ballot(inst == -1);
+        if (need != 0) {
+            const int leader = __ffsll((unsigned long long)need) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&ws.counters[0], (uint32_t)__popcll(need));
+            base = __shfl(base, leader);
+            if (inst == -1) {
+                const uint64_t k = (uint64_t)base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
+                if (k < n) {
+                    inst = (int64_t)k;
+                    const uint64_t i = k;
+#pragma unroll
+                    for (int c = 0; c < D; ++c) L.y[c] = d.y0[c * n + i];
+#pragma unroll
+                    for (int c = 0; c < RHS::P; ++c) L.p[c] = d.params[c * n + i];
+                    L.t = d.t0 ? d.t0[i] : d.t0_s;
+                    L.end = d.t_end ? d.t_end[i] : d.t_end_s;
+                    L.tol = d.tol ? d.tol[i] : d.tol_s;
+                    L.dt_min = d.dt_min ? d.dt_min[i] : d.dt_min_s;
+                    L.dt_max = d.dt_max ? d.dt_max[i] : d.dt_max_s;
+                    L.dt = (L.dt_max + L.dt_min) * 0.5;
+                    attempts = nacc = nrej = next_stop = 0;
+                    n_stops = count_stops(obs, n_obs, L.end);
+                    loss = 0.0;
+                    // checkpoint 0: the initial state
+                    ws.ck[(0 * F + 0) * n + i] = L.t;
+                    ws.ck[(0 * F + 1) * n + i] = L.dt;
+                    ws.ck[(0 * F + 2) * n + i] = 0.0;
+#pragma unroll
+                    for (int c = 0; c < D; ++c) ws.ck[(0 * F + 3 + c) * n + i] = L.y[c];
+                    nck = 1;
+                } else {
+                    inst = -2;
+                }
+            }
+        }
+        if (__ballot(inst >= 0) == 0) break;
+        int st = -1;
+        for (;;) {
+            if (__ballot(inst >= 0 && st >= 0) != 0) break;
+            if (inst >= 0 && st < 0) {
+                if (L.t >= L.end) {
+                    st = BG_OK;
+                } else if (attempts >= kAdjMaxAttempts) {
+                    st = BG_MAXIMUM_ITERATIONS_EXCEEDED;
+                } else {
+                    const bool to_stop = next_stop < n_stops;
+                    const double target = to_stop ? obs[next_stop] : L.end;
+                    bool clamped;
+                    const int r = L.attempt_to(target, clamped);
+                    ++attempts;
+                    if (r == STEP_ACCEPTED) {
+                        ++nacc;
+                        const uint64_t i = (uint64_t)inst;
+                        if (clamped && to_stop) {
+                            L.t = target;
+#pragma unroll
+                            for (int c = 0; c < D; ++c) {
+                                const uint64_t o = ((uint64_t)next_stop * D + c) * n + i;
+                                const double rr = L.y[c] - ad->targets[o];
+                                ws.res[o] = rr;
+                                loss = fma(rr, rr, loss);
+                                if (ad->obs_y) ad->obs_y[o] = L.y[c];
+                            }
+                            ++next_stop;
+                        }
+                        if (L.t >= L.end) {
+                            st = BG_OK;
+                        } else if (nacc % K == 0) {
+                            if (nck >= maxck) {
+                                st = BG_MAXIMUM_ITERATIONS_EXCEEDED;
+                            } else {
+                                ws.ck[((uint64_t)nck * F + 0) * n + i] = L.t;
+                                ws.ck[((uint64_t)nck * F + 1) * n + i] = L.dt;
+                                ws.ck[((uint64_t)nck * F + 2) * n + i] = (double)next_stop;
+#pragma unroll
+                                for (int c = 0; c < D; ++c)
+                                    ws.ck[((uint64_t)nck * F + 3 + c) * n + i] = L.y[c];
+                                ++nck;
+                            }
+                        }
+                    } else if (r == STEP_REJECTED) {
+                        ++nrej;
+                    } else {
+                        st = r;
+                    }
+                }
+            }
+        }
+        if (inst >= 0 && st >= 0) {
+            const uint64_t i = (uint64_t)inst;
+            // an observation at the end time (after the last stop) scores the
+            // final state
+            if (st == BG_OK && n_stops < n_obs && obs[n_stops] == L.end) {
+#pragma unroll
+                for (int c = 0; c < D; ++c) {
+                    const uint64_t o = ((uint64_t)n_stops * D + c) * n + i;
+                    const double rr = L.y[c] - ad->targets[o];
+                    ws.res[o] = rr;
+                    loss = fma(rr, rr, loss);
+                    if (ad->obs_y) ad->obs_y[o] = L.y[c];
+                }
+            }
+            ws.nck[i] = nck;
+            if (ad->loss) ad->loss[i] = loss;
+            if (d.t_out) d.t_out[i] = L.t;
+            if (d.y_out) {
+#pragma unroll
+                for (int c = 0; c < D; ++c) d.y_out[c * n + i] = L.y[c];
+            }
+            if (d.status) d.status[i] = st;
+            if (d.counts) {
+                bg_counts cc;
+                cc.n_accepted = nacc;
+                cc.n_rejected = nrej;
+                d.counts[i] = cc;
+            }
+            inst = -1;
+        }
+    }
+}
+
+// Reverse of one accepted step of the explicit map
+//   Y_i = y + sum_{j<i} a_ij K_j,  K_i = dt f(Y_i),  y' = y + sum_i b_i K_i
+// given lam = dL/dy'; returns dL/dy in lam and accumulates dL/dp into pb.
+template <class RHS, class Tab>
+__device__ __forceinline__ void vjp_step(double t, double dt, const double (&y)[RHS::D],
+                                         const double *p, double (&lam)[RHS::D], double *pb) {
+    constexpr int D = RHS::D, S = Tab::S;
+    double Y[S][D], K[S][D], Kb[S][D];
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) Y[i][c] = y[c];
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            const double m = Tab::m(i, j);
+            if (m != 0.0) {
+#pragma unroll
+                for (int c = 0; c < D; ++c) Y[i][c] = fma(m, K[j][c], Y[i][c]);
+            }
+        }
+        double f[D];
+        RHS::eval(fma(Tab::c(i), dt, t), Y[i], p, f);
+#pragma unroll
+        for (int c = 0; c < D; ++c) K[i][c] = f[c] * dt;
+    }
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+#pragma unroll
+        for (int c = 0; c < D; ++c) Kb[i][c] = Tab::b(i) * lam[c];
+    double ysum[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) ysum[c] = 0.0;
+#pragma unroll
+    for (int i = S - 1; i >= 0; --i) {
+        double fb[D], yb[D];
+#pragma unroll
+        for (int c = 0; c < D; ++c) fb[c] = Kb[i][c] * dt;
+        RHS::vjp(fma(Tab::c(i), dt, t), Y[i], p, fb, yb, pb);
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            const double m = Tab::m(i, j);
+            if (m != 0.0) {
+#pragma unroll
+                for (int c = 0; c < D; ++c) Kb[j][c] = fma(m, yb[c], Kb[j][c]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < D; ++c) ysum[c] += yb[c];
+    }
+#pragma unroll
+    for (int c = 0; c < D; ++c) lam[c] += ysum[c];
+}
+
+template <class RHS, class Tab>
+__global__ void __launch_bounds__(256)
+    rk_adjoint_backward_kernel(const bg_adjoint_desc *__restrict__ ad, const AdjWs ws) {
+    constexpr int D = RHS::D, F = 3 + D, P = RHS::P > 0 ? RHS::P : 1;
+    const bg_ensemble_desc &d = ad->base;
+    const int lane = __lane_id();
+    const uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t n = d.n, S = ws.slots;
+    const uint32_t K = ad->checkpoint_every ? ad->checkpoint_every : 16u;
+    const uint32_t n_obs = ad->n_obs;
+    const double *obs = ad->obs_t;
+    RkLane<RHS, Tab> L;
+    double lam[D], pb[P];
+    int64_t inst = -1;
+    int seg = -1;
+    for (;;) {
+        const uint64_t need = __ballot(inst == -1);
+        if (need != 0) {
+            const int leader = __ffsll((unsigned long long)need) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&ws.counters[1], (uint32_t)__popcll(need));
+            base = __shfl(base, leader);
+            if (inst == -1) {
+                const uint64_t k = (uint64_t)base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
+                if (k < n) {
+                    inst = (int64_t)k;
+                    const uint64_t i = k;
+#pragma unroll
+                    for (int c = 0; c < RHS::P; ++c) L.p[c] = d.params[c * n + i];
+                    L.end = d.t_end ? d.t_end[i] : d.t_end_s;
+                    L.tol = d.tol ? d.tol[i] : d.tol_s;
+                    L.dt_min = d.dt_min ? d.dt_min[i] : d.dt_min_s;
+                    L.dt_max = d.dt_max ? d.dt_max[i] : d.dt_max_s;
+#pragma unroll
+                    for (int c = 0; c < P; ++c) pb[c] = 0.0;
+                    const uint32_t ns = count_stops(obs, n_obs, L.end);
+                    const bool final_obs = ns < n_obs && obs[ns] == L.end;
+#pragma unroll
+                    for (int c = 0; c < D; ++c)
+                        lam[c] = final_obs ? 2.0 * ws.res[((uint64_t)ns * D + c) * n + i] : 0.0;
+                    const int fw = d.status ? d.status[i] : BG_OK;
+                    seg = fw == BG_OK ? (int)ws.nck[i] - 1 : -1;
+                } else {
+                    inst = -2;
+                }
+            }
+        }
+        if (__ballot(inst >= 0) == 0) break;
+        bool fin = false;
+        for (;;) {
+            if (__ballot(inst >= 0 && fin) != 0) break;
+            if (inst >= 0 && !fin) {
+                if (seg < 0) {
+                    fin = true;
+                } else {
+                    const uint64_t i = (uint64_t)inst;
+                    // ---- recompute segment `seg` from its checkpoint ----
+                    L.t = ws.ck[((uint64_t)seg * F + 0) * n + i];
+                    L.dt = ws.ck[((uint64_t)seg * F + 1) * n + i];
+                    uint32_t next_stop = (uint32_t)ws.ck[((uint64_t)seg * F + 2) * n + i];
+#pragma unroll
+                    for (int c = 0; c < D; ++c) L.y[c] = ws.ck[((uint64_t)seg * F + 3 + c) * n + i];
+                    const uint32_t n_stops = count_stops(obs, n_obs, L.end);
+                    uint32_t cnt = 0, guard = 0;
+                    while (cnt < K && L.t < L.end && guard < kAdjMaxAttempts) {
+                        ++guard;
+                        const bool to_stop = next_stop < n_stops;
+                        const double target = to_stop ? obs[next_stop] : L.end;
+                        const double t_before = L.t;
+                        double y_before[D];
+#pragma unroll
+                        for (int c = 0; c < D; ++c) y_before[c] = L.y[c];
+                        const double dt_used = (L.t + L.dt >= target) ? target - L.t : L.dt;
+                        bool clamped;
+                        const int r = L.attempt_to(target, clamped);
+                        if (r == STEP_ACCEPTED) {
+                            double stop_idx = -1.0;
+                            if (clamped && to_stop) {
+                                L.t = target;
+                                stop_idx = (double)next_stop;
+                                ++next_stop;
+                            }
+                            double *rec = ws.seg + ((uint64_t)cnt * F) * S + slot;
+                            rec[0 * S] = t_before;
+                            rec[1 * S] = dt_used;
+                            rec[2 * S] = stop_idx;
+#pragma unroll
+                            for (int c = 0; c < D; ++c) rec[(3 + c) * S] = y_before[c];
+                            ++cnt;
+                        } else if (r != STEP_REJECTED) {
+                            break;  // cannot happen on a replay of a successful forward
+                        }
+                    }
+                    // ---- reverse sweep ----
+                    for (int k = (int)cnt - 1; k >= 0; --k) {
+                        const double *rec = ws.seg + ((uint64_t)k * F) * S + slot;
+                        const double tk = rec[0 * S], dk = rec[1 * S], sk = rec[2 * S];
+                        double yk[D];
+#pragma unroll
+                        for (int c = 0; c < D; ++c) yk[c] = rec[(3 + c) * S];
+                        if (sk >= 0.0) {
+                            const uint64_t s = (uint64_t)sk;
+#pragma unroll
+                            for (int c = 0; c < D; ++c)
+                                lam[c] = fma(2.0, ws.res[(s * D + c) * n + i], lam[c]);
+                        }
+                        vjp_step<RHS, Tab>(tk, dk, yk, L.p, lam, pb);
+                    }
+                    --seg;
+                    if (seg < 0) fin = true;
+                }
+            }
+        }
+        if (inst >= 0 && fin) {
+            const uint64_t i = (uint64_t)inst;
+            const int fw = d.status ? d.status[i] : BG_OK;
+            const double nan = __builtin_nan("");
+            if (ad->grad_y0) {
+#pragma unroll
+                for (int c = 0; c < D; ++c) ad->grad_y0[c * n + i] = fw == BG_OK ? lam[c] : nan;
+            }
+            if (ad->grad_params) {
+#pragma unroll
+                for (int c = 0; c < RHS::P; ++c)
+                    ad->grad_params[c * n + i] = fw == BG_OK ? pb[c] : nan;
+            }
+            inst = -1;
+        }
+    }
+}
+
+__global__ void adj_setup_kernel(uint32_t *counters, bg_adjoint_desc *dst, const bg_adjoint_desc d) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        counters[0] = 0u;
+        counters[1] = 0u;
+        *dst = d;
+    }
+}
+
+template <class T>
+struct ATag {
+    using type = T;
+};
+
+int rk_launch_adjoint(int method, int rhs, const bg_adjoint_desc &ad, hipStream_t stream) {
+    if (method != BG_RK45) return BG_ERR_UNSUPPORTED;
+    const bg_ensemble_desc &d = ad.base;
+    auto run = [&](auto rtag) -> int {
+        using R = typename decltype(rtag)::type;
+        using T = Tableau<Fehlberg45, false>;
+        constexpr int D = R::D, F = 3 + D;
+        if (d.dim != (uint32_t)D) return BG_ERR_INVALID_ARGUMENT;
+        if (d.n == 0) return BG_OK;
+        if (d.n > 0x7fffffffull || !d.y0 || !d.params || d.n_params < (uint32_t)R::P ||
+            !d.status || (ad.n_obs && (!ad.obs_t || !ad.targets)))
+            return BG_ERR_INVALID_ARGUMENT;
+        const uint64_t n = d.n;
+        const uint64_t K = ad.checkpoint_every ? ad.checkpoint_every : 16u;
+        const uint64_t maxck = ad.max_checkpoints ? ad.max_checkpoints : 128u;
+        int dev = 0;
+        BG_HIP_CHECK(hipGetDevice(&dev));
+        const int cus = device_cu_count(dev);
+        int per_cu_f = 0, per_cu_b = 0;
+        BG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu_f, rk_adjoint_forward_kernel<R, T>, 256, 0));
+        BG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu_b, rk_adjoint_backward_kernel<R, T>, 256, 0));
+        per_cu_f = per_cu_f < 1 ? 1 : per_cu_f;
+        per_cu_b = per_cu_b < 1 ? 1 : per_cu_b;
+        uint64_t grid_f = (uint64_t)per_cu_f * cus, grid_b = (uint64_t)per_cu_b * cus;
+        const uint64_t need = (n + 255) / 256;
+        if (grid_f > need) grid_f = need;
+        if (grid_b > need) grid_b = need;
+        const uint64_t slots = grid_b * 256;
+        const uint64_t n_res = (uint64_t)(ad.n_obs ? ad.n_obs : 1) * D * n;
+        const size_t bytes = 256 + sizeof(bg_adjoint_desc) + 64 +
+                             sizeof(double) * (maxck * F * n + n_res + K * F * slots) +
+                             sizeof(uint32_t) * n + 256;
+        char *buf = nullptr;
+        BG_HIP_CHECK(hipMallocAsync((void **)&buf, bytes, stream));
+        AdjWs ws;
+        ws.counters = (uint32_t *)buf;
+        bg_adjoint_desc *dd = (bg_adjoint_desc *)(buf + 256);
+        char *p = buf + 256 + sizeof(bg_adjoint_desc);
+        p += (64 - ((uintptr_t)p % 64)) % 64;
+        ws.ck = (double *)p;
+        ws.res = ws.ck + maxck * F * n;
+        ws.seg = ws.res + n_res;
+        ws.nck = (uint32_t *)(ws.seg + K * F * slots);
+        ws.slots = slots;
+        adj_setup_kernel<<<1, 64, 0, stream>>>(ws.counters, dd, ad);
+        rk_adjoint_forward_kernel<R, T><<<(uint32_t)grid_f, 256, 0, stream>>>(dd, ws);
+        rk_adjoint_backward_kernel<R, T><<<(uint32_t)grid_b, 256, 0, stream>>>(dd, ws);
+        BG_HIP_CHECK(hipGetLastError());
+        BG_HIP_CHECK(hipFreeAsync(buf, stream));
+        return BG_OK;
+    };
+    switch (rhs) {
+    case BG_RHS_LOTKA_VOLTERRA:
+        return run(ATag<RhsLotkaVolterra>{});
+    case BG_RHS_LORENZ:
+        return run(ATag<RhsLorenz>{});
+    default:
+        return BG_ERR_UNSUPPORTED;
+    }
+}
+
+}  // namespace bg

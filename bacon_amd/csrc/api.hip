@@ -657,6 +657,11 @@ extern "C" int bg_bdf_solve_device(int method, int rhs, int semantics, int lu_bi
     return bdf_launch_ensemble(method, rhs, semantics, lu_bits == 32, *d, (hipStream_t)stream);
 }
 
+extern "C" int bg_rk_vjp_device(int method, int rhs, const bg_adjoint_desc *d, void *stream) {
+    if (!d || method != BG_RK45) return BG_ERR_INVALID_ARGUMENT;
+    return rk_launch_adjoint(method, rhs, *d, (hipStream_t)stream);
+}
+
 extern "C" int bg_version(void) { return 1; }
 
 extern "C" int bg_device_info(int *n_cu, int *arch_major, int *arch_minor) {

@@ -137,17 +137,41 @@ struct RhsLorenz {
         f[2] = fma(y[0], y[1], -(b * y[2]));
         return true;
     }
+    static constexpr int VJP_FLOPS = 21;
+    __device__ static void vjp(double, const double (&y)[3], const double *p, const double (&fb)[3],
+                               double (&yb)[3], double *pb) {
+        const double s = p[0], r = p[1], b = p[2];
+        yb[0] = fma(-s, fb[0], fma(r - y[2], fb[1], y[1] * fb[2]));
+        yb[1] = fma(s, fb[0], fma(y[0], fb[2], -fb[1]));
+        yb[2] = fma(-y[0], fb[1], -(b * fb[2]));
+        pb[0] = fma(y[1] - y[0], fb[0], pb[0]);
+        pb[1] = fma(y[0], fb[1], pb[1]);
+        pb[2] = fma(-y[2], fb[2], pb[2]);
+    }
 };
 
 struct RhsLotkaVolterra {
     static constexpr int D = 2, P = 4;
     static constexpr int FLOPS = 7;
+    static constexpr int VJP_FLOPS = 20;
     __device__ static bool eval(double, const double (&y)[2], const double *p, double (&f)[2]) {
         const double a = p[0], b = p[1], g = p[2], d = p[3];
         const double xy = y[0] * y[1];
         f[0] = fma(a, y[0], -(b * xy));
         f[1] = fma(d, xy, -(g * y[1]));
         return true;
+    }
+    // yb = (df/dy)^T fb, pb += (df/dp)^T fb
+    __device__ static void vjp(double, const double (&y)[2], const double *p, const double (&fb)[2],
+                               double (&yb)[2], double *pb) {
+        const double a = p[0], b = p[1], g = p[2], d = p[3];
+        const double x = y[0], w = y[1], xw = x * w;
+        yb[0] = fma(a - b * w, fb[0], (d * w) * fb[1]);
+        yb[1] = fma(-(b * x), fb[0], (d * x - g) * fb[1]);
+        pb[0] = fma(x, fb[0], pb[0]);
+        pb[1] = fma(-xw, fb[0], pb[1]);
+        pb[2] = fma(-w, fb[1], pb[2]);
+        pb[3] = fma(xw, fb[1], pb[3]);
     }
 };
 
@@ -203,8 +227,19 @@ struct RkLane {
     // rk.rs:361-423 without the Done test (the caller does it).  Returns
     // STEP_ACCEPTED, STEP_REJECTED or an IVPError code.
     __device__ __forceinline__ int attempt() {
-        // rk.rs:366-368
-        if (t + dt >= end) dt = end - t;
+        bool clamped;
+        return attempt_to(end, clamped);
+    }
+
+    // The same attempt with the end clamp of rk.rs:366-368 aimed at `target`
+    // (the end, or an observation time before it); `clamped` reports whether
+    // dt was cut to reach it.
+    __device__ __forceinline__ int attempt_to(double target, bool &clamped) {
+        clamped = false;
+        if (t + dt >= target) {
+            dt = target - t;
+            clamped = true;
+        }
         // rk.rs:370-384
 #pragma unroll
         for (int i = 0; i < S; ++i) {

@@ -210,6 +210,34 @@ int bg_rk_solve_plain_device(int method, int rhs, int semantics, const bg_ensemb
 int bg_bdf_solve_device(int method, int rhs, int semantics, int lu_bits,
                         const bg_ensemble_desc *d, void *stream);
 
+/* Discrete adjoint of the accepted RK step sequence (RK45, intended tableau)
+ * for the loss L = sum_k |y(t_k) - target_k|^2 over observation times t_k
+ * shared by all instances (increasing, in (t0, t_end]; observation times
+ * before t_end become stops: the step clamp of rk.rs:366-368 aims at the
+ * next one and the state lands exactly on it).  The forward solve records a
+ * checkpoint (t, dt, next stop, y) every `checkpoint_every` accepted steps;
+ * the backward pass re-runs each segment from its checkpoint (bit-identical
+ * to the forward) and applies the reverse stage-by-stage VJP of the frozen
+ * step map.  Gradients are with respect to the per-instance parameters and
+ * y0.  No counterpart in the reference (it has no sensitivity API). */
+typedef struct {
+    bg_ensemble_desc base;      /* inputs; status, counts, t_out, y_out (final) */
+    uint32_t n_obs;
+    uint32_t checkpoint_every;  /* K (0 = 16)                                   */
+    uint32_t max_checkpoints;   /* per instance (0 = 128)                       */
+    uint32_t pad;
+    const double *obs_t;        /* [n_obs]                                      */
+    const double *targets;      /* [n_obs][dim][n]                              */
+    double *loss;               /* [n]                                          */
+    double *grad_params;        /* [n_params][n]                                */
+    double *grad_y0;            /* [dim][n]                                     */
+    double *obs_y;              /* [n_obs][dim][n] states at the observation
+                                   times, or NULL                              */
+} bg_adjoint_desc;
+/* Loss and its gradient (vector-Jacobian product with a unit cotangent per
+ * instance) on `stream`, asynchronous. */
+int bg_rk_vjp_device(int method, int rhs, const bg_adjoint_desc *d, void *stream);
+
 /* Library / device information */
 int bg_version(void);
 int bg_device_info(int *n_cu, int *arch_major, int *arch_minor);

@@ -1,0 +1,150 @@
+"""GPU discrete adjoint (bg_rk_vjp_device) against independent references.
+
+  * forward pass with observation stops: bit-identical to the oracle's
+    DEVICE-profile solve with the same stops (states at every observation);
+  * gradients vs central finite differences (h = 1e-6) of the loss evaluated
+    by replaying the frozen accepted-step sequence in the oracle: relative
+    error below 1e-6 per instance (the step sequence comes from the oracle's
+    own forward solve, which the first test shows equals the GPU's);
+  * gradients vs the oracle's forward-sensitivity solve of the augmented
+    14-dimensional Lotka-Volterra system at tol 1e-12: the continuous and
+    discrete sensitivities differ by the discretisation error, so the bar is
+    1e-4 relative;
+  * the torch.autograd.Function plumbing and the checkpoint-capacity error.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from bacon_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-8
+
+
+def _t(a, dtype=None):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a),
+                           dtype=dtype or torch.float64, device="cuda:0")
+
+
+@pytest.fixture(scope="module")
+def lv():
+    """8 Lotka-Volterra instances of the adjoint workload, targets = the GPU
+    solution at the perturbed parameters."""
+    import torch
+    from bacon_amd.adjoint import solve_vjp
+    w = W.lotka_volterra(8)
+    obs = w["obs"]
+    zeros = torch.zeros((obs.size, 2, 8), dtype=torch.float64, device="cuda:0")
+    r = solve_vjp("lotka_volterra", _t(w["y0"]), _t(w["target_params"]), _t(obs), zeros,
+                  15.0, TOL)
+    assert torch.all(r["status"] == 0)
+    w["targets"] = r["obs_y"].cpu().numpy()
+    return w
+
+
+def _gpu(w, **kw):
+    from bacon_amd.adjoint import solve_vjp
+    r = solve_vjp("lotka_volterra", _t(w["y0"]), _t(w["params"]), _t(w["obs"]),
+                  _t(w["targets"]), 15.0, TOL, **kw)
+    return {k: v.cpu().numpy() for k, v in r.items()}
+
+
+def _oracle_path(w, i, params=None):
+    p = w["params"][:, i] if params is None else params
+    return O.solve(O.RK45, "lotka_volterra", w["y0"][:, i], 0.0, 15.0, TOL, 1e-12, 0.1,
+                   params=p, semantics=O.INTENDED, profile=O.PROFILE_DEVICE,
+                   stops=w["obs"][:-1], path_cap=100000)
+
+
+def test_forward_with_stops_bit_exact(gpu, orc, lv):
+    g = _gpu(lv)
+    assert np.all(g["status"] == 0)
+    for i in range(8):
+        r = _oracle_path(lv, i)
+        assert r["status"] == 0
+        assert np.array_equal(g["obs_y"][:-1, :, i], r["stop_y"])
+        assert np.array_equal(g["obs_y"][-1, :, i], r["y"])
+        assert g["counts"][i, 0] == r["n_accepted"]
+        loss = ((np.vstack([r["stop_y"], r["y"][None]]) - lv["targets"][:, :, i]) ** 2).sum()
+        assert abs(g["loss"][i] - loss) <= 1e-12 * max(1.0, loss)
+
+
+def _frozen_loss(w, i, r, params, y0):
+    """Loss of the frozen step sequence of path r replayed from (params, y0)."""
+    t_before = np.concatenate([[0.0], r["path_t"][:-1]])
+    rc, ys = O.rk_replay("lotka_volterra", y0, t_before, r["path_dt"], params=params,
+                         profile=O.PROFILE_DEVICE)
+    assert rc == 0
+    obs = w["obs"]
+    loss = 0.0
+    for k, tk in enumerate(obs[:-1]):
+        idx = np.nonzero(r["path_t"] == tk)[0]
+        assert idx.size == 1
+        loss += ((ys[idx[0]] - w["targets"][k, :, i]) ** 2).sum()
+    loss += ((ys[-1] - w["targets"][-1, :, i]) ** 2).sum()
+    return loss
+
+
+def test_vjp_vs_frozen_step_finite_differences(gpu, orc, lv):
+    g = _gpu(lv)
+    h = 1e-6
+    for i in range(8):
+        r = _oracle_path(lv, i)
+        p0, y00 = lv["params"][:, i].copy(), lv["y0"][:, i].copy()
+        base = _frozen_loss(lv, i, r, p0, y00)
+        assert abs(base - g["loss"][i]) <= 1e-12 * max(1.0, base)
+        fd = np.zeros(6)
+        for k in range(6):
+            e = np.zeros(6)
+            e[k] = h
+            lp = _frozen_loss(lv, i, r, p0 + e[:4], y00 + e[4:])
+            lm = _frozen_loss(lv, i, r, p0 - e[:4], y00 - e[4:])
+            fd[k] = (lp - lm) / (2 * h)
+        adj = np.concatenate([g["grad_params"][:, i], g["grad_y0"][:, i]])
+        rel = np.linalg.norm(adj - fd) / np.linalg.norm(fd)
+        assert rel < 1e-6, (i, rel, adj, fd)
+
+
+def test_vjp_vs_forward_sensitivity(gpu, orc, lv):
+    g = _gpu(lv)
+    for i in range(8):
+        z0 = np.zeros(14)
+        z0[:2] = lv["y0"][:, i]
+        z0[10] = 1.0  # dx/dx0
+        z0[13] = 1.0  # dy/dy0
+        r = O.solve(O.RK45, "lv_sens", z0, 0.0, 15.0, 1e-12, 1e-14, 0.1,
+                    params=lv["params"][:, i], semantics=O.INTENDED, stops=lv["obs"][:-1])
+        assert r["status"] == 0
+        zs = np.vstack([r["stop_y"], r["y"][None]])  # [n_obs, 14]
+        grad = np.zeros(6)
+        for k in range(zs.shape[0]):
+            res = zs[k, :2] - lv["targets"][k, :, i]
+            S = zs[k, 2:].reshape(6, 2).T  # S[:, col]
+            grad += 2.0 * res @ S
+        adj = np.concatenate([g["grad_params"][:, i], g["grad_y0"][:, i]])
+        rel = np.linalg.norm(adj - grad) / np.linalg.norm(grad)
+        assert rel < 1e-4, (i, rel)
+
+
+def test_autograd_function(gpu, lv):
+    import torch
+    from bacon_amd.adjoint import RK45Loss
+    params = _t(lv["params"]).requires_grad_(True)
+    y0 = _t(lv["y0"]).requires_grad_(True)
+    loss = RK45Loss.apply(params, y0, "lotka_volterra", _t(lv["obs"]), _t(lv["targets"]),
+                          15.0, TOL)
+    weights = torch.arange(1, 9, dtype=torch.float64, device="cuda:0")
+    (loss * weights).sum().backward()
+    g = _gpu(lv)
+    w = np.arange(1, 9)
+    assert np.allclose(params.grad.cpu().numpy(), g["grad_params"] * w, rtol=1e-14, atol=0)
+    assert np.allclose(y0.grad.cpu().numpy(), g["grad_y0"] * w, rtol=1e-14, atol=0)
+
+
+def test_checkpoint_capacity_exceeded(gpu, lv):
+    g = _gpu(lv, max_checkpoints=2)
+    assert np.all(g["status"] == 9)
+    assert np.all(np.isnan(g["grad_params"]))
