@@ -1,14 +1,21 @@
-"""Benchmark: batched RK45 Lorenz-63 ensemble on one MI355X.
+"""Benchmark: batched RK45 Lorenz-63 ensemble on one MI355X (+ adjoint, BDF6).
 
-A step is one batched solve of the whole ensemble: 1,048,576 Lorenz-63
-instances (sigma=10, beta=8/3, rho~U[20,40], y0~U[-15,15]^2 x U[5,40],
-t_end~U[1,20], tol log-uniform in [1e-10,1e-4], dt_max 0.1, dt_min 1e-12),
-FP64, one persistent launch, inputs resident in HBM before timing starts.
+Headline step: one batched solve of 1,048,576 Lorenz-63 instances
+(sigma=10, beta=8/3, rho~U[20,40], y0~U[-15,15]^2 x U[5,40], t_end~U[1,20],
+tol log-uniform in [1e-10,1e-4], dt_max 0.1, dt_min 1e-12), FP64, one
+persistent launch, inputs resident in HBM before the timed region.
 
-Prints one JSON line: solves/sec (value), accepted steps/sec, the kernel's
-FP64 roofline fraction measured with HIP events on the launch stream, the
-plain per-instance GPU path on a subset of the same workload, and the CPU
-oracle timed on the host cores over a bounded sample.
+The JSON line also carries:
+  * roofline of the dominant kernel (HIP events on its launch stream) and the
+    HBM bytes measured by rocprofv3 (see profiles/);
+  * the plain per-instance GPU path on a subset of the same workload;
+  * the same ensemble at a uniform tol 1e-8 (the metric's tolerance), with
+    its own plain-path comparison;
+  * adjoint gradients/sec: discrete adjoint of 262,144 Lotka-Volterra
+    instances (20 observations, checkpoint every 16 accepted steps);
+  * batched BDF6 on 131,072 stiff Brusselator instances, Jacobian inverse in
+    FP64 and in FP32;
+  * the CPU oracle timed on the host cores over a bounded sample.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1]
 """
@@ -29,12 +36,21 @@ sys.path.insert(0, ROOT)
 METRIC = "RK45 ensemble solves/sec at tol 1e-8 vs plain GPU path; adjoint grads/sec"
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix peaks are equal)
 PEAK_HBM_GBS = 8000.0
-# algorithmic FLOPs of one RK45 step attempt on Lorenz-63 (fma = 2):
+# algorithmic FLOPs of one RK45 step attempt (fma = 2):
 # 2D(nnz(A)+nnz(e)+nnz(b)) + 6(F_rhs + D) + (2D + 2) + F_ctrl
-#   = 2*3*(15+5+4) + 6*(8+3) + 8 + 6 = 224
-FLOPS_PER_ATTEMPT = 224
-# HBM bytes per instance: loads y0(3) rho sigma beta t_end tol (+order) and
-# stores t, y(3), status, counts
+FLOPS_ATTEMPT_LORENZ = 2 * 3 * (15 + 5 + 4) + 6 * (8 + 3) + 8 + 6   # 224
+FLOPS_ATTEMPT_LV = 2 * 2 * (15 + 5 + 4) + 6 * (7 + 2) + 6 + 6       # 162
+# reverse of one accepted LV step: stage recompute (6 RHS, 15 nnz(A) combos,
+# 6 dt scalings) + 6 VJPs + stage-adjoint combos + b-weighting + sums
+FLOPS_VJP_STEP_LV = (6 * 7 + 2 * 2 * 15 + 6 * 2) + 6 * 20 + 2 * 2 * 15 + 2 * 2 * 4 + 6 * 2 * 2
+# BDF6 Brusselator (N = 32): full RHS 304 flops; residual = RHS + 15 N
+BRUSS_N = 32
+F_RHS_BRUSS = 16 * 19
+F_RESID = F_RHS_BRUSS + 15 * BRUSS_N
+FLOPS_SECANT_SETUP = (1 + 2 * BRUSS_N) * F_RESID + 2 * BRUSS_N ** 3 + 2 * BRUSS_N ** 2
+FLOPS_BROYDEN_ITER = F_RESID + 5 * 2 * BRUSS_N ** 2 + 6 * BRUSS_N
+# HBM bytes per Lorenz instance: loads y0(3) rho sigma beta t_end tol, order
+# index; stores t, y(3), status, counts
 BYTES_PER_INSTANCE = 8 * 8 + 4 + 8 * 4 + 4 + 8
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE and WRITE_SIZE (each
 # scaled by a 1 GiB copy-kernel calibration); see
@@ -44,6 +60,75 @@ MEASURED_TRAFFIC_BYTES = 1.054e9 + 0.238e9
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+class Ctx:
+    def __init__(self, torch, L, dev):
+        self.torch, self.L, self.dev = torch, L, dev
+        self.lib = L.lib()
+        self.stream = torch.cuda.current_stream()
+        self.sh = C.c_void_p(self.stream.cuda_stream)
+
+    def T(self, a, dt=None):
+        torch = self.torch
+        return torch.as_tensor(np.ascontiguousarray(a), dtype=dt or torch.float64,
+                               device=self.dev)
+
+    def zeros(self, shape, dt=None):
+        return self.torch.zeros(shape, dtype=dt or self.torch.float64, device=self.dev)
+
+
+def rk_ensemble(cx, w, order=True):
+    """Device buffers and descriptor for a Lorenz workload dict."""
+    from bacon_amd import workloads as W
+    torch = cx.torch
+    n = w["y0"].shape[1]
+    b = dict(y0=cx.T(w["y0"]), params=cx.T(w["params"]), tend=cx.T(w["t_end"]),
+             tol=cx.T(w["tol"]), t_out=cx.zeros(n), y_out=cx.zeros((3, n)),
+             status=cx.zeros(n, torch.int32), counts=cx.zeros((n, 2), torch.int32))
+    if order:
+        b["order"] = cx.T(W.cost_order(w["t_end"], w["tol"]), torch.int32)
+    d = cx.L.EnsembleDesc(
+        n=n, dim=3, n_params=3, y0=b["y0"].data_ptr(), params=b["params"].data_ptr(),
+        t0_s=0.0, t_end=b["tend"].data_ptr(), tol=b["tol"].data_ptr(), dt_min_s=w["dt_min"],
+        dt_max_s=w["dt_max"], order=b["order"].data_ptr() if order else None,
+        t_out=b["t_out"].data_ptr(), y_out=b["y_out"].data_ptr(),
+        status=b["status"].data_ptr(), counts=b["counts"].data_ptr())
+    return b, d
+
+
+def time_launches(cx, fn, steps, warmup):
+    torch = cx.torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(steps)]
+    t0 = time.perf_counter()
+    for s, e in ev:
+        s.record(cx.stream)
+        fn()
+        e.record(cx.stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    return wall, float(np.mean([s.elapsed_time(e) for s, e in ev])) / 1000.0
+
+
+def plain_path(cx, w, m):
+    """The plain per-instance path on the first m instances of workload w."""
+    n = w["y0"].shape[1]
+    sub = {k: (v[..., :m] if isinstance(v, np.ndarray) and v.ndim and v.shape[-1] == n else v)
+           for k, v in w.items()}
+    b, d = rk_ensemble(cx, sub, order=False)
+    cx.torch.cuda.synchronize()
+    t = time.perf_counter()
+    rc = cx.lib.bg_rk_solve_plain_device(cx.L.RK45, cx.L.RHS["lorenz"], cx.L.INTENDED,
+                                         C.byref(d), cx.sh)
+    cx.torch.cuda.synchronize()
+    t = time.perf_counter() - t
+    if rc != 0:
+        raise RuntimeError(f"plain path returned {rc}")
+    return m / t, b
 
 
 def main():
@@ -57,6 +142,8 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the headline ensemble (used for profiling)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the solve, run a plain device copy of this many MiB "
@@ -75,41 +162,17 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(0)
-    lib = L.lib()
+    cx = Ctx(torch, L, dev)
     n = args.n
     log(f"[bench] generating workload n={n}")
     w = W.lorenz_batched(n)
-    order_np = W.cost_order(w["t_end"], w["tol"])
-
-    def T(a, dt=torch.float64):
-        return torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
-
-    y0, params = T(w["y0"]), T(w["params"])
-    tend, tol = T(w["t_end"]), T(w["tol"])
-    order = T(order_np, torch.int32)
-    t_out = torch.zeros(n, dtype=torch.float64, device=dev)
-    y_out = torch.zeros((3, n), dtype=torch.float64, device=dev)
-    status = torch.zeros(n, dtype=torch.int32, device=dev)
-    counts = torch.zeros((n, 2), dtype=torch.int32, device=dev)
-
-    def desc(nn, order_ptr=None):
-        return L.EnsembleDesc(
-            n=nn, dim=3, n_params=3, y0=y0.data_ptr(), params=params.data_ptr(),
-            t0_s=0.0, t_end=tend.data_ptr(), tol=tol.data_ptr(), dt_min_s=w["dt_min"],
-            dt_max_s=w["dt_max"], order=order_ptr, t_out=t_out.data_ptr(),
-            y_out=y_out.data_ptr(), status=status.data_ptr(), counts=counts.data_ptr())
-
-    # a subset descriptor must address the first nn instances with stride nn;
-    # the full descriptor uses stride n.
-    d_full = desc(n, order.data_ptr())
+    b, d_full = rk_ensemble(cx, w)
     launch = L.Launch(block=args.block, blocks_per_cu=args.blocks_per_cu,
                       refill_min=args.refill_min)
-    stream = torch.cuda.current_stream()
-    sh = C.c_void_p(stream.cuda_stream)
 
     def run_once():
-        rc = lib.bg_rk_solve_device(L.RK45, L.RHS["lorenz"], L.INTENDED, C.byref(d_full),
-                                    C.byref(launch), sh)
+        rc = cx.lib.bg_rk_solve_device(L.RK45, L.RHS["lorenz"], L.INTENDED, C.byref(d_full),
+                                       C.byref(launch), cx.sh)
         if rc != 0:
             raise RuntimeError(f"bg_rk_solve_device returned {rc}")
 
@@ -122,37 +185,18 @@ def main():
         log(f"[bench] copy calibration: {nb} bytes read, {nb} bytes written")
         del src, dst
 
-    log(f"[bench] warmup {args.warmup}")
-    for _ in range(args.warmup):
-        run_once()
-    torch.cuda.synchronize()
+    log(f"[bench] headline: warmup {args.warmup}, {args.steps} timed steps")
+    wall, kern = time_launches(cx, run_once, args.steps, args.warmup)
+    ms_per_step = wall * 1000.0
+    log(f"[bench] ms/step {ms_per_step:.3f} (events {kern * 1e3:.3f})")
 
-    log(f"[bench] timing {args.steps} steps")
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for s, e in ev:
-        s.record(stream)
-        run_once()
-        e.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kernel_ms = [s.elapsed_time(e) for s, e in ev]
-    ms_per_step = wall * 1000.0 / args.steps
-    log(f"[bench] ms/step {ms_per_step:.3f} (events {np.mean(kernel_ms):.3f})")
-
-    st = status.cpu().numpy()
-    cnt = counts.cpu().numpy().astype(np.int64)
-    n_acc = int(cnt[:, 0].sum())
-    n_rej = int(cnt[:, 1].sum())
+    st = b["status"].cpu().numpy()
+    cnt = b["counts"].cpu().numpy().astype(np.int64)
+    n_acc, n_rej = int(cnt[:, 0].sum()), int(cnt[:, 1].sum())
     attempts = n_acc + n_rej
-    n_fail = int(np.count_nonzero(st))
     per_inst = cnt[:, 0] + cnt[:, 1]
-    solves_per_s = n / (ms_per_step / 1000.0)
-    avg_kernel_s = float(np.mean(kernel_ms)) / 1000.0
-    flops = attempts * FLOPS_PER_ATTEMPT
-    achieved_tflops = flops / avg_kernel_s / 1e12
+    solves_per_s = n / wall
+    achieved = attempts * FLOPS_ATTEMPT_LORENZ / kern / 1e12
 
     result = {
         "metric": METRIC,
@@ -173,66 +217,146 @@ def main():
                         "dt_max 0.1, dt_min 1e-12, one persistent launch",
             "n_instances": n,
             "semantics": "intended",
-            "parallelism": "one lane per instance, persistent grid, wave-ballot refill",
+            "parallelism": "one lane per instance, persistent grid, wave-ballot refill, "
+                           "longest-first schedule",
         },
-        "accepted_steps_per_s": n_acc / (ms_per_step / 1000.0),
+        "accepted_steps_per_s": n_acc / wall,
         "attempts_per_launch": attempts,
         "attempts_per_instance": {"mean": float(per_inst.mean()),
                                   "p99": float(np.percentile(per_inst, 99)),
                                   "max": int(per_inst.max())},
-        "failed_instances": n_fail,
+        "failed_instances": int(np.count_nonzero(st)),
         "roofline": {
             "bound": "mfma",
             "compute": "fp64 (MI355X fp64 vector peak = fp64 matrix peak = 78.6 TFLOP/s; "
                        "this kernel issues FP64 VALU fma)",
-            "achieved": achieved_tflops,
+            "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": achieved_tflops / PEAK_FP64_TFLOPS,
+            "frac": achieved / PEAK_FP64_TFLOPS,
             "traffic": MEASURED_TRAFFIC_BYTES,
             "kernel": "rk_ensemble_kernel<RhsLorenz, Tableau<Fehlberg45,false>>",
-            "flops_per_attempt": FLOPS_PER_ATTEMPT,
-            "avg_kernel_ms": float(np.mean(kernel_ms)),
+            "flops_per_attempt": FLOPS_ATTEMPT_LORENZ,
+            "avg_kernel_ms": kern * 1e3,
             "algorithmic_hbm_bytes": n * BYTES_PER_INSTANCE,
         },
     }
 
-    # plain per-instance path on a subset of the same workload
     if not args.no_plain:
         m = 64
-        sub = W.lorenz_batched(m)
-        ys, ps = T(sub["y0"]), T(sub["params"])
-        te, tl = T(sub["t_end"]), T(sub["tol"])
-        to = torch.zeros(m, dtype=torch.float64, device=dev)
-        yo = torch.zeros((3, m), dtype=torch.float64, device=dev)
-        so = torch.zeros(m, dtype=torch.int32, device=dev)
-        co = torch.zeros((m, 2), dtype=torch.int32, device=dev)
-        ds = L.EnsembleDesc(n=m, dim=3, n_params=3, y0=ys.data_ptr(), params=ps.data_ptr(),
-                            t_end=te.data_ptr(), tol=tl.data_ptr(), dt_min_s=w["dt_min"],
-                            dt_max_s=w["dt_max"], t_out=to.data_ptr(), y_out=yo.data_ptr(),
-                            status=so.data_ptr(), counts=co.data_ptr())
-        log("[bench] plain per-instance path, 64 instances")
-        torch.cuda.synchronize()
-        tp = time.perf_counter()
-        rc = lib.bg_rk_solve_plain_device(L.RK45, L.RHS["lorenz"], L.INTENDED, C.byref(ds), sh)
-        torch.cuda.synchronize()
-        tp = time.perf_counter() - tp
-        if rc != 0:
-            raise RuntimeError(f"plain path returned {rc}")
-        plain_sps = m / tp
-        # same instances in the batched result must be identical
-        same = bool(np.array_equal(yo.cpu().numpy(), y_out[:, :m].cpu().numpy()))
+        log(f"[bench] plain per-instance path, {m} instances")
+        plain_sps, pb = plain_path(cx, w, m)
+        same = bool(np.array_equal(pb["y_out"].cpu().numpy(),
+                                   b["y_out"][:, :m].cpu().numpy()))
         result["plain_gpu"] = {"solves_per_s": plain_sps, "sample": f"first {m} instances",
                                "identical_to_batched": same}
         result["speedup_vs_plain"] = solves_per_s / plain_sps
 
+    if not args.no_extras:
+        # ---- the metric's tolerance: uniform tol 1e-8 ---------------------------
+        log("[bench] uniform tol 1e-8 ensemble")
+        wu = dict(w)
+        wu["tol"] = np.full(n, 1e-8)
+        bu, du = rk_ensemble(cx, wu)
+
+        def run_u():
+            rc = cx.lib.bg_rk_solve_device(L.RK45, L.RHS["lorenz"], L.INTENDED, C.byref(du),
+                                           C.byref(launch), cx.sh)
+            if rc != 0:
+                raise RuntimeError(rc)
+        wall_u, kern_u = time_launches(cx, run_u, 2, 1)
+        cu = bu["counts"].cpu().numpy().astype(np.int64)
+        tol8 = {"solves_per_s": n / wall_u, "ms_per_solve": wall_u * 1e3,
+                "accepted_steps_per_s": int(cu[:, 0].sum()) / wall_u,
+                "fp64_tflops": int(cu.sum()) * FLOPS_ATTEMPT_LORENZ / kern_u / 1e12}
+        if not args.no_plain:
+            plain_u, pbu = plain_path(cx, wu, 64)
+            tol8["plain_gpu_solves_per_s"] = plain_u
+            tol8["speedup_vs_plain"] = (n / wall_u) / plain_u
+            tol8["identical_to_batched"] = bool(np.array_equal(
+                pbu["y_out"].cpu().numpy(), bu["y_out"][:, :64].cpu().numpy()))
+        result["uniform_tol_1e-8"] = tol8
+        del bu, du
+
+        # ---- adjoint gradients --------------------------------------------------
+        from bacon_amd.adjoint import solve_vjp
+        na = 262144
+        log(f"[bench] adjoint, {na} Lotka-Volterra instances")
+        wl = W.lotka_volterra(na)
+        obs = cx.T(wl["obs"])
+        zeros = cx.zeros((wl["obs"].size, 2, na))
+        tr = solve_vjp("lotka_volterra", cx.T(wl["y0"]), cx.T(wl["target_params"]), obs, zeros,
+                       15.0, 1e-8)
+        targets = tr["obs_y"].clone()
+        del tr
+        y0l, pl = cx.T(wl["y0"]), cx.T(wl["params"])
+        res = {}
+
+        def run_adj():
+            res["r"] = solve_vjp("lotka_volterra", y0l, pl, obs, targets, 15.0, 1e-8)
+        wall_a, kern_a = time_launches(cx, run_adj, 3, 1)
+        r = res["r"]
+        ca = r["counts"].cpu().numpy().astype(np.int64)
+        acc_a, att_a = int(ca[:, 0].sum()), int(ca.sum())
+        # forward attempts + backward recompute attempts + reverse steps
+        flops_a = 2 * att_a * FLOPS_ATTEMPT_LV + acc_a * FLOPS_VJP_STEP_LV
+        result["adjoint_grads_per_s"] = na / wall_a
+        result["adjoint"] = {
+            "workload": "discrete adjoint of batched RK45, 262,144 Lotka-Volterra instances, "
+                        "tol 1e-8, t in [0,15], 20 observations, checkpoint every 16 steps; "
+                        "gradients w.r.t. 4 parameters and y0",
+            "grads_per_s": na / wall_a, "ms_per_solve": wall_a * 1e3,
+            "failed_instances": int(np.count_nonzero(r["status"].cpu().numpy())),
+            "accepted_steps": acc_a,
+            "fp64_tflops": flops_a / kern_a / 1e12,
+        }
+        del r, res
+
+        # ---- BDF6 Brusselator ---------------------------------------------------
+        nb_ = 131072
+        log(f"[bench] BDF6, {nb_} Brusselator instances")
+        wb = W.brusselator(nb_)
+        bb = dict(y0=cx.T(wb["y0"]), params=cx.T(wb["params"]), t_out=cx.zeros(nb_),
+                  y_out=cx.zeros((32, nb_)), status=cx.zeros(nb_, torch.int32),
+                  counts=cx.zeros((nb_, 2), torch.int32), work=cx.zeros((nb_, 2), torch.int32))
+        db = L.EnsembleDesc(
+            n=nb_, dim=32, n_params=3, y0=bb["y0"].data_ptr(), params=bb["params"].data_ptr(),
+            t0_s=0.0, t_end_s=10.0, tol_s=1e-7, dt_min_s=1e-12, dt_max_s=0.1,
+            t_out=bb["t_out"].data_ptr(), y_out=bb["y_out"].data_ptr(),
+            status=bb["status"].data_ptr(), counts=bb["counts"].data_ptr(),
+            work=bb["work"].data_ptr())
+        bdf = {"workload": "batched BDF6 (intended), 131,072 1-D Brusselator instances "
+                           "(16 cells x 2 species), B log-uniform [2,4], tol 1e-7, t in [0,10]"}
+        y64 = None
+        for bits in (64, 32):
+            def run_b():
+                rc = cx.lib.bg_bdf_solve_device(L.BDF6, L.RHS["brusselator"], L.INTENDED, bits,
+                                                C.byref(db), cx.sh)
+                if rc != 0:
+                    raise RuntimeError(rc)
+            wall_b, kern_b = time_launches(cx, run_b, 1, 1)
+            wk = bb["work"].cpu().numpy().astype(np.int64)
+            flops_b = (int(wk[:, 1].sum()) * FLOPS_SECANT_SETUP
+                       + int(wk[:, 0].sum()) * FLOPS_BROYDEN_ITER)
+            yb = bb["y_out"].cpu().numpy()
+            entry = {"solves_per_s": nb_ / wall_b, "ms_per_solve": wall_b * 1e3,
+                     "failed_instances": int(np.count_nonzero(bb["status"].cpu().numpy())),
+                     "accepted_steps": int(bb["counts"].cpu().numpy()[:, 0].sum()),
+                     "broyden_iterations": int(wk[:, 0].sum()),
+                     "jacobian_inverses": int(wk[:, 1].sum()),
+                     "tflops": flops_b / kern_b / 1e12}
+            if bits == 64:
+                y64 = yb
+            else:
+                entry["max_abs_diff_vs_fp64_lu"] = float(np.abs(yb - y64).max())
+            bdf[f"lu_fp{bits}"] = entry
+        result["bdf6_brusselator"] = bdf
+
     if not args.no_cpu_baseline:
         import oracle as O
-        threads = os.cpu_count() or 1
-        threads = min(threads, 16)
+        threads = min(os.cpu_count() or 1, 16)
         log(f"[bench] CPU oracle baseline ({threads} threads, ~{args.cpu_seconds}s)")
-        done = 0
-        chunk = 512
+        done, chunk = 0, 512
         tc = time.perf_counter()
         while time.perf_counter() - tc < args.cpu_seconds and done < n:
             s = W.lorenz_batched(chunk, first=done)
