@@ -43,6 +43,23 @@ def _compile(src: str) -> str:
     return out
 
 
+CPP_TEST_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "test_reference_ivp.cpp")
+CPP_TEST_BIN = os.path.join(os.path.dirname(HERE), "tests", "cpp", "test_reference_ivp.bin")
+
+
+def build_cpp_tests(verbose: bool = True) -> str:
+    """The C++ restatement of the reference's ivp tests, linked against the
+    library (runs on a GPU box from tests/test_gpu_cpp.py)."""
+    cmd = ["g++", "-std=c++17", "-O2", "-o", CPP_TEST_BIN, CPP_TEST_SRC,
+           f"-L{HERE}", "-lbacon_gpu", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../bacon_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {CPP_TEST_BIN}")
+    return CPP_TEST_BIN
+
+
 def build(verbose: bool = True) -> str:
     os.makedirs(OBJ, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
@@ -55,6 +72,7 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_cpp_tests(verbose)
     return LIB
 
 
