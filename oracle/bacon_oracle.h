@@ -21,6 +21,13 @@
  *                       sqrt(sqrt(x)).  Every operation is an IEEE correctly
  *                       rounded primitive on both CPU and GPU, so the
  *                       single-lane GPU RK kernels must match it bit for bit.
+ *
+ * Pinning (tests/test_oracle.py): the reference's own ivp tests, order
+ * conditions, closed forms, scipy golden vectors, LAPACK inverses.
+ * Parity unpinned: the exact rounding of nalgebra 0.32's multi-dimensional
+ * norm / LU / full-pivot LU / QR (restated from its published algorithms; the
+ * crate is not vendored and cannot be built here) and as-built step counts for
+ * y-dependent right-hand sides.
  */
 #ifndef BACON_ORACLE_H
 #define BACON_ORACLE_H
