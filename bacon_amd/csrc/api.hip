@@ -481,6 +481,9 @@ extern "C" int bg_ensemble_with_derivative(bg_ensemble *e, int rhs, const double
     if (P < 0) return BG_ERR_UNSUPPORTED;
     if (n_params < (uint32_t)P || (n_params && !params) || (n_shared && !shared))
         return BG_ERR_INVALID_ARGUMENT;
+    // the linear right-hand side reads A (dim x dim, row-major) then b
+    if (rhs == BG_RHS_LINEAR && n_shared < (uint64_t)e->dim * e->dim + e->dim)
+        return BG_ERR_INVALID_ARGUMENT;
     e->rhs = rhs;
     e->n_params = n_params;
     e->params.assign(params, params + (size_t)n_params * e->n);
@@ -546,10 +549,10 @@ static int ensemble_prepare(bg_ensemble *e, bg_ensemble_desc &d, bool want_order
     if ((rc = upload_field(e->d_dtmin, e->dt_min, &d.dt_min, &d.dt_min_s)) != BG_OK) return rc;
     if ((rc = upload_field(e->d_dtmax, e->dt_max, &d.dt_max, &d.dt_max_s)) != BG_OK) return rc;
     if (want_order && e->n > 1 && (!e->t_end.v.empty() || !e->tol.v.empty() || !e->t0.v.empty())) {
-        // longest-first schedule: predicted cost (t_end - t0) * tol^(-1/5)
+        // longest-first schedule: predicted cost (t_end - t0) * tol^(-1/4)
         std::vector<double> cost(e->n);
         for (uint64_t i = 0; i < e->n; ++i)
-            cost[i] = (e->t_end.at(i) - e->t0.at(i)) * pow(e->tol.at(i), -0.2);
+            cost[i] = (e->t_end.at(i) - e->t0.at(i)) * pow(e->tol.at(i), -0.25);
         std::vector<int32_t> order(e->n);
         std::iota(order.begin(), order.end(), 0);
         std::stable_sort(order.begin(), order.end(),

@@ -16,6 +16,11 @@ int rk_launch_plain(int method, int rhs, int semantics, const bg_ensemble_desc &
                     hipStream_t stream);
 bool rk_supported(int method, int rhs, uint32_t dim);
 
+// RK45 for y' = A y + b with one shared A (dim 128), the stage right-hand
+// sides as FP64 MFMA GEMMs over tiles of 16 instances (linear_kernels.hip).
+int linear_launch_ensemble(int method, int semantics, const bg_ensemble_desc &d,
+                           hipStream_t stream);
+
 // Single-instance path production (one lane).  The state block lives in
 // device memory and is resumable: the kernel stops when the path buffer is
 // full and the host relaunches it after draining the buffer.

@@ -322,6 +322,8 @@ static int check_desc(const bg_ensemble_desc &d, int P) {
 
 int rk_launch_ensemble(int method, int rhs, int semantics, const bg_ensemble_desc &d,
                        const bg_launch &lc, hipStream_t stream) {
+    // the shared-matrix linear right-hand side runs as a tiled MFMA GEMM kernel
+    if (rhs == BG_RHS_LINEAR) return linear_launch_ensemble(method, semantics, d, stream);
     return visit_rk(method, rhs, semantics, d.dim, [&](auto rtag, auto ttag) -> int {
         using R = typename decltype(rtag)::type;
         using T = typename decltype(ttag)::type;

@@ -121,7 +121,8 @@ def linear_shared(n: int = 65536, first: int = 0, N: int = 128):
     """Linear y' = A y + b with one shared A = -(Q D Q^T): Q the sign-fixed
     orthogonal factor of a seeded N(0,1) matrix, D log-uniform in [0.1,10],
     b ~ N(0,1) shared; y0 ~ N(0,1); t in [0,5], tol 1e-9 (workload index 2).
-    Also returns the eigen-data for the closed-form solution."""
+    Also returns the eigen-data for the closed-form solution and the shared
+    block [A row-major; b] the linear right-hand side reads."""
     g = Stream(2, N * N + 2 * N, 0)  # shared draws use stream indices 0..
     flat = g.normal()
     M = flat[: N * N].reshape(N, N)
@@ -134,11 +135,12 @@ def linear_shared(n: int = 65536, first: int = 0, N: int = 128):
     y0 = np.stack([s.normal() for _ in range(N)])
     return dict(rhs="linear", dim=N, y0=y0, params=np.zeros((0, n)), t0=0.0,
                 t_end=np.full(n, 5.0), tol=np.full(n, 1e-9), dt_min=1e-12,
-                dt_max=0.1, A=A, b=bvec, Q=Q, D=Dv)
+                dt_max=0.1, A=A, b=bvec, Q=Q, D=Dv,
+                shared=np.concatenate([A.reshape(-1), bvec]))
 
 
 def cost_order(t_end: np.ndarray, tol: np.ndarray, t0: float = 0.0) -> np.ndarray:
     """Longest-first (LPT) schedule: instances sorted by predicted cost
-    (t_end - t0) * tol^(-1/5), descending."""
-    cost = (t_end - t0) * np.power(tol, -0.2)
+    (t_end - t0) * tol^(-1/4), descending."""
+    cost = (t_end - t0) * np.power(tol, -0.25)
     return np.argsort(-cost, kind="stable").astype(np.int32)

@@ -15,6 +15,8 @@ The JSON line also carries:
     instances (20 observations, checkpoint every 16 accepted steps);
   * batched BDF6 on 131,072 stiff Brusselator instances, Jacobian inverse in
     FP64 and in FP32;
+  * batched RK45 on 65,536 linear systems (N = 128) sharing one matrix, the
+    stage right-hand sides as FP64 MFMA GEMMs;
   * the CPU oracle timed on the host cores over a bounded sample.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1]
@@ -49,6 +51,10 @@ F_RHS_BRUSS = 16 * 19
 F_RESID = F_RHS_BRUSS + 15 * BRUSS_N
 FLOPS_SECANT_SETUP = (1 + 2 * BRUSS_N) * F_RESID + 2 * BRUSS_N ** 3 + 2 * BRUSS_N ** 2
 FLOPS_BROYDEN_ITER = F_RESID + 5 * 2 * BRUSS_N ** 2 + 6 * BRUSS_N
+# linear y' = A y + b, N = 128, one shared A: RHS = matvec (2 N^2) + b (N)
+LIN_N = 128
+FLOPS_ATTEMPT_LINEAR = (2 * LIN_N * (15 + 5 + 4) + 6 * (2 * LIN_N * LIN_N + LIN_N + LIN_N)
+                        + 2 * LIN_N + 2 + 6)
 # HBM bytes per Lorenz instance: loads y0(3) rho sigma beta t_end tol, order
 # index; stores t, y(3), status, counts
 BYTES_PER_INSTANCE = 8 * 8 + 4 + 8 * 4 + 4 + 8
@@ -351,6 +357,55 @@ def main():
                 entry["max_abs_diff_vs_fp64_lu"] = float(np.abs(yb - y64).max())
             bdf[f"lu_fp{bits}"] = entry
         result["bdf6_brusselator"] = bdf
+        del bb, db
+
+        # ---- linear shared-A RK45 (FP64 MFMA tiles) -----------------------------
+        nl = 65536
+        log(f"[bench] linear shared-A RK45, {nl} instances, N={LIN_N}")
+        wl = W.linear_shared(nl)
+        bl = dict(y0=cx.T(wl["y0"]), shared=cx.T(wl["shared"]), t_out=cx.zeros(nl),
+                  y_out=cx.zeros((LIN_N, nl)), status=cx.zeros(nl, torch.int32),
+                  counts=cx.zeros((nl, 2), torch.int32))
+        dl = L.EnsembleDesc(
+            n=nl, dim=LIN_N, n_params=0, y0=bl["y0"].data_ptr(), shared=bl["shared"].data_ptr(),
+            t0_s=0.0, t_end_s=5.0, tol_s=1e-9, dt_min_s=wl["dt_min"], dt_max_s=wl["dt_max"],
+            t_out=bl["t_out"].data_ptr(), y_out=bl["y_out"].data_ptr(),
+            status=bl["status"].data_ptr(), counts=bl["counts"].data_ptr())
+
+        def run_l():
+            rc = cx.lib.bg_rk_solve_device(L.RK45, L.RHS["linear"], L.INTENDED, C.byref(dl),
+                                           None, cx.sh)
+            if rc != 0:
+                raise RuntimeError(rc)
+        wall_l, kern_l = time_launches(cx, run_l, 2, 1)
+        cl = bl["counts"].cpu().numpy().astype(np.int64)
+        yl = bl["y_out"].cpu().numpy()
+        xs = np.linalg.solve(wl["A"], -wl["b"])
+        ye = xs[:, None] + wl["Q"] @ (np.exp(-wl["D"] * 5.0)[:, None]
+                                      * (wl["Q"].T @ (wl["y0"] - xs[:, None])))
+        lin = {"workload": "batched RK45 (intended), 65,536 instances of y' = A y + b, "
+                           "one shared 128x128 A = -Q diag(D) Q^T, D log-uniform [0.1,10], "
+                           "tol 1e-9, t in [0,5]; stage right-hand sides as FP64 MFMA GEMMs",
+               "solves_per_s": nl / wall_l, "ms_per_solve": wall_l * 1e3,
+               "failed_instances": int(np.count_nonzero(bl["status"].cpu().numpy())),
+               "attempts": int(cl.sum()),
+               "fp64_tflops": int(cl.sum()) * FLOPS_ATTEMPT_LINEAR / kern_l / 1e12,
+               "frac_of_fp64_peak": int(cl.sum()) * FLOPS_ATTEMPT_LINEAR / kern_l / 1e12
+                                    / PEAK_FP64_TFLOPS,
+               "max_abs_err_vs_closed_form": float(np.abs(yl - ye).max())}
+        if not args.no_cpu_baseline:
+            import oracle as O
+            m, threads = 64, min(os.cpu_count() or 1, 16)
+            tcl = time.perf_counter()
+            O.solve_ensemble(O.RK45, "linear", wl["y0"][:, :m], None, wl["t_end"][:m],
+                             wl["tol"][:m], dt_min=wl["dt_min"], dt_max=wl["dt_max"],
+                             shared=wl["shared"], semantics=O.INTENDED,
+                             profile=O.PROFILE_BACON, threads=threads)
+            tcl = time.perf_counter() - tcl
+            lin["cpu_oracle"] = {"solves_per_s": m / tcl, "cores": threads,
+                                 "sample": f"first {m} instances"}
+        result["linear_shared_mfma"] = lin
+        del bl, dl
 
     if not args.no_cpu_baseline:
         import oracle as O

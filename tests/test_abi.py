@@ -144,3 +144,12 @@ def test_cost_order_longest_first():
     tol = np.array([1e-4, 1e-4, 1e-10])
     o = W.cost_order(t_end, tol)
     assert list(o) == [2, 1, 0]
+
+
+def test_ensemble_linear_shared_block_size(lib):
+    """The linear right-hand side needs the whole [A; b] block (dim^2 + dim)."""
+    e = ivp.Ensemble(L.RK45, 128, 2)
+    with pytest.raises(ivp.IVPError) as ex:
+        e.with_derivative("linear", shared=np.zeros(128 * 128))
+    assert ex.value.name == "InvalidArgument"
+    e.with_derivative("linear", shared=np.zeros(128 * 128 + 128))
