@@ -48,6 +48,7 @@ template <class RHS, class Tab>
 __global__ void __launch_bounds__(256)
     rk_adjoint_forward_kernel(const bg_adjoint_desc *__restrict__ ad, const AdjWs ws) {
     constexpr int D = RHS::D, F = 3 + D;
+    r8_init();
     const bg_ensemble_desc &d = ad->base;
     const int lane = __lane_id();
     const uint64_t n = d.n;
@@ -239,6 +240,7 @@ template <class RHS, class Tab>
 __global__ void __launch_bounds__(256)
     rk_adjoint_backward_kernel(const bg_adjoint_desc *__restrict__ ad, const AdjWs ws) {
     constexpr int D = RHS::D, F = 3 + D, P = RHS::P > 0 ? RHS::P : 1;
+    r8_init();
     const bg_ensemble_desc &d = ad->base;
     const int lane = __lane_id();
     const uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;

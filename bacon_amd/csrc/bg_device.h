@@ -19,6 +19,23 @@
 namespace bg {
 
 // ---------------------------------------------------------------------------
+// Coefficients as FMA operands.  A VOP2 v_fmac_f64 can take a 32-bit literal
+// but overwrites its addend, so a literal coefficient applied to a value
+// that stays live (y in Y = y + m K) costs an extra v_mov_b64; a VOP3
+// v_fma_f64 cannot encode a literal on gfx950 but takes an SGPR pair.  kc()
+// passes a coefficient that is not an inline constant through an SGPR pair
+// (an empty asm the optimiser cannot see into), so the three-address form is
+// used.  The value is unchanged.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double kc(double v) {
+    if (v == 0.5 || v == -0.5 || v == 1.0 || v == -1.0 || v == 2.0 || v == -2.0 || v == 4.0 ||
+        v == -4.0)
+        return v;
+    asm("" : "+s"(v));
+    return v;
+}
+
+// ---------------------------------------------------------------------------
 // Tableaux.  L is the reference's literal "Row i" data (rk.rs:459-502,
 // 580-601).  The reference stores it with Matrix::from_vec, which fills
 // column-major, and iterates row_iter, so its stage loop reads L transposed;
@@ -193,22 +210,50 @@ struct RhsOscillator {
 // out.  Used as the step-size factor (tol/err)^(1/4); the caller bounds it
 // by the tangent at r = 1.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double root8(double r) {
+// 2^((e mod 8)/8) as the product ((s1 s2) s4) of the selected constants,
+// rounded exactly as a runtime evaluation would (constexpr arithmetic is
+// IEEE round-to-nearest).
+__host__ __device__ constexpr double root8_scale(int b) {
+    return (((b & 1) ? 1.0905077326652577 : 1.0) * ((b & 2) ? 1.189207115002721 : 1.0)) *
+           ((b & 4) ? 1.4142135623730951 : 1.0);
+}
+
+// The eight scales in LDS for kernels that call root8_lds(): a per-lane
+// table read replaces three select pairs and two products.  r8_init() fills
+// it at kernel entry (every thread of the block must call it).
+__shared__ double r8_table[8];
+__device__ __forceinline__ void r8_init() {
+    if (threadIdx.x < 8) r8_table[threadIdx.x] = root8_scale((int)threadIdx.x);
+    __syncthreads();
+}
+
+__device__ __forceinline__ double root8_poly(double r, int &e) {
     int ex;
     const double mant = frexp(r, &ex);
-    const double x = mant * 2.0 - 1.5;
-    const int e = ex - 1;
-    double p = 0.0029682975037312034;
-    p = fma(p, x, -0.005693672682686368);
-    p = fma(p, x, 0.010621966427993313);
-    p = fma(p, x, -0.025512534910039072);
-    p = fma(p, x, 0.08766623673998887);
-    p = fma(p, x, 1.0519887150881322);
-    const int a = e >> 3, b = e & 7;
-    const double s1 = (b & 1) ? 1.0905077326652577 : 1.0;
-    const double s2 = (b & 2) ? 1.189207115002721 : 1.0;
-    const double s4 = (b & 4) ? 1.4142135623730951 : 1.0;
-    return ldexp(p * ((s1 * s2) * s4), a);
+    // mant * 2 is exact, so the fused form rounds like mul-then-add
+    const double x = fma(mant, 2.0, -1.5);
+    e = ex - 1;
+    // coefficients through SGPRs: three-address fma, no accumulator copies
+    double p = kc(0.0029682975037312034);
+    p = fma(p, x, kc(-0.005693672682686368));
+    p = fma(p, x, kc(0.010621966427993313));
+    p = fma(p, x, kc(-0.025512534910039072));
+    p = fma(p, x, kc(0.08766623673998887));
+    p = fma(p, x, kc(1.0519887150881322));
+    return p;
+}
+
+__device__ __forceinline__ double root8(double r) {
+    int e;
+    const double p = root8_poly(r, e);
+    return ldexp(p * root8_scale(e & 7), e >> 3);
+}
+
+// The same value with the scale read from r8_table.
+__device__ __forceinline__ double root8_lds(double r) {
+    int e;
+    const double p = root8_poly(r, e);
+    return ldexp(p * r8_table[e & 7], e >> 3);
 }
 
 // ---------------------------------------------------------------------------
@@ -251,7 +296,7 @@ struct RkLane {
                 const double m = Tab::m(i, j);
                 if (m != 0.0) {
 #pragma unroll
-                    for (int d = 0; d < D; ++d) Y[d] = fma(m, K[j][d], Y[d]);
+                    for (int d = 0; d < D; ++d) Y[d] = fma(kc(m), K[j][d], Y[d]);
                 }
             }
             const double ti = fma(Tab::c(i), dt, t);
@@ -264,7 +309,7 @@ struct RkLane {
         double E[D];
         constexpr int e0 = Tab::e_first();
 #pragma unroll
-        for (int d = 0; d < D; ++d) E[d] = Tab::e(e0) * K[e0][d];
+        for (int d = 0; d < D; ++d) E[d] = kc(Tab::e(e0)) * K[e0][d];
 #pragma unroll
         for (int j = e0 + 1; j < S; ++j) {
             const double e = Tab::e(j);
@@ -284,16 +329,17 @@ struct RkLane {
         const double q2 = q * q;
         const bool accepted = ss <= q2;
         const double r = q2 / ss;
-        const double f0 = r <= 1e-8 ? 0.1 : (r >= 65536.0 ? 4.0 : root8(r));
+        const double f0 = r <= 1e-8 ? 0.1 : (r >= 65536.0 ? 4.0 : root8_lds(r));
         // near r = 1 the sign of delta - 1 ~ (r - 1)/8 is far below root8's
         // error: bound by the tangent 1 + (r - 1)/8 (r^(1/8) is concave, the
         // tangent is exact to second order at 1), then keep the exact power's
-        // side of 1 (a rejection shrinks dt, an acceptance never does); NaN
-        // passes through
+        // side of 1 (a rejection shrinks dt, an acceptance never does).
+        // fmin/fmax are IEEE minNum/maxNum: a NaN error estimate (rejected)
+        // gets the factor 1 - 2^-53, so dt stays finite and the instance runs
+        // into the attempt cap (the reference retries forever).
         const double tang = fma(r - 1.0, 0.125, 1.0);
-        const double f = f0 < tang ? f0 : tang;
-        const double factor = accepted ? (f < 1.0 ? 1.0 : f)
-                                       : (f > 0x1.fffffffffffffp-1 ? 0x1.fffffffffffffp-1 : f);
+        const double f = fmin(f0, tang);
+        const double factor = accepted ? fmax(f, 1.0) : fmin(f, 0x1.fffffffffffffp-1);
         // rk.rs:392-398
         if (accepted) {
             t += dt;
@@ -306,8 +352,8 @@ struct RkLane {
                 }
             }
         }
-        dt *= factor;
-        if (dt > dt_max) dt = dt_max;
+        // dt is never NaN here (factor is finite), so the clamp is a minimum
+        dt = fmin(dt * factor, dt_max);
         // rk.rs:414-416
         if (dt < dt_min && t < end) return BG_MINIMUM_TIME_DELTA_EXCEEDED;
         return accepted ? STEP_ACCEPTED : STEP_REJECTED;

@@ -192,9 +192,8 @@ __global__ void __launch_bounds__(256)
                 const double rr = q2 / ss;
                 const double f0 = rr <= 1e-8 ? 0.1 : (rr >= 65536.0 ? 4.0 : root8(rr));
                 const double tang = fma(rr - 1.0, 0.125, 1.0);
-                const double f = f0 < tang ? f0 : tang;
-                const double factor = accepted ? (f < 1.0 ? 1.0 : f)
-                                               : (f > 0x1.fffffffffffffp-1 ? 0x1.fffffffffffffp-1 : f);
+                const double f = fmin(f0, tang);
+                const double factor = accepted ? fmax(f, 1.0) : fmin(f, 0x1.fffffffffffffp-1);
                 if (accepted) {
                     t += dt;
 #pragma unroll
@@ -208,8 +207,7 @@ __global__ void __launch_bounds__(256)
                         }
                     }
                 }
-                dt *= factor;
-                if (dt > dt_max) dt = dt_max;
+                dt = fmin(dt * factor, dt_max);
                 // rk.rs:414-416; an accepted step that then fails the dt_min
                 // test is reported as the failure, like the generic kernels
                 if (dt < dt_min && t < end) {

@@ -128,6 +128,7 @@ template <class RHS, class Tab>
 __global__ void __launch_bounds__(256)
     rk_ensemble_kernel(const volatile bg_ensemble_desc *dv, uint32_t *__restrict__ counter,
                        uint32_t refill_min, uint32_t prio_cutoff) {
+    r8_init();
     RkLane<RHS, Tab> L;
     const int lane = __lane_id();
     const uint64_t max_att = dv->max_attempts;
@@ -184,6 +185,7 @@ __global__ void __launch_bounds__(256)
 // Plain path: one instance per launch, one lane.
 template <class RHS, class Tab>
 __global__ void __launch_bounds__(64) rk_plain_kernel(const bg_ensemble_desc d, uint64_t k) {
+    r8_init();
     if (threadIdx.x != 0) return;
     RkLane<RHS, Tab> L;
     const uint64_t i = d.order ? (uint64_t)d.order[k] : k;
@@ -205,6 +207,7 @@ __global__ void __launch_bounds__(64)
     rk_path_kernel(PathState *__restrict__ st, double *__restrict__ path_t,
                    double *__restrict__ path_y, uint32_t cap) {
     constexpr int D = RHS::D, S = Tab::S;
+    r8_init();
     if (threadIdx.x != 0) return;
     RkLane<RHS, Tab> L;
     L.t = st->t;

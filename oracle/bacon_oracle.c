@@ -428,10 +428,14 @@ static int rk_step(rk_solver *s, double *used_dt) {
          * 1 + (r - 1)/8, bounds it from above and is exact to second order
          * there: take the smaller of the two.  Then keep the exact power's
          * side of 1 (delta < 1 on a rejection, >= 1 on an acceptance) against
-         * the final rounding.  NaN passes through. */
+         * the final rounding.  fmin/fmax (IEEE minNum/maxNum) are single
+         * instructions on the device; they differ from compare-and-select only
+         * for NaN: a NaN error estimate (rejected, like the reference) gets the
+         * factor 1 - 2^-53 instead of NaN, so dt stays finite and the instance
+         * ends at the attempt cap either way (the reference retries forever). */
         const double tang = fma(r - 1.0, 0.125, 1.0);
-        const double f = f0 < tang ? f0 : tang;
-        factor = accepted ? (f < 1.0 ? 1.0 : f) : (f > 0x1.fffffffffffffp-1 ? 0x1.fffffffffffffp-1 : f);
+        const double f = fmin(f0, tang);
+        factor = accepted ? fmax(f, 1.0) : fmin(f, 0x1.fffffffffffffp-1);
     } else {
         const double error = nal_norm(D, scratch) / s->dt;
         accepted = error <= s->tol;
