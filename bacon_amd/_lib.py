@@ -53,6 +53,25 @@ class EnsembleDesc(C.Structure):
     ]
 
 
+TUNE_MAX_PARAMS = 8
+
+
+class Tuning(C.Structure):
+    _fields_ = [
+        ("launch", Launch), ("n_params", C.c_uint32), ("pilot_n", C.c_uint32),
+        ("coef", C.c_double * (3 + TUNE_MAX_PARAMS)),
+        ("pilot_ms", C.c_double), ("default_ms", C.c_double), ("best_ms", C.c_double),
+        ("n_trials", C.c_uint32), ("pad", C.c_uint32),
+    ]
+
+    def as_dict(self):
+        return {"block": self.launch.block, "blocks_per_cu": self.launch.blocks_per_cu,
+                "coef": [float(c) for c in self.coef[: 3 + self.n_params]],
+                "pilot_n": self.pilot_n, "pilot_ms": self.pilot_ms,
+                "default_ms": self.default_ms, "best_ms": self.best_ms,
+                "n_trials": self.n_trials}
+
+
 class AdjointDesc(C.Structure):
     _fields_ = [
         ("base", EnsembleDesc), ("n_obs", C.c_uint32), ("checkpoint_every", C.c_uint32),
@@ -103,6 +122,12 @@ SIGNATURES = [
     ("bg_ensemble_destroy", None, [C.c_void_p]),
     ("bg_rk_solve_device", C.c_int,
      [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.POINTER(Launch), C.c_void_p]),
+    ("bg_rk_autotune_device", C.c_int,
+     [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.POINTER(Tuning), C.c_void_p]),
+    ("bg_rk_schedule_device", C.c_int,
+     [C.POINTER(Tuning), C.POINTER(EnsembleDesc), C.c_void_p, C.c_void_p]),
+    ("bg_rk_solve_tuned_device", C.c_int,
+     [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.POINTER(Tuning), C.c_void_p]),
     ("bg_rk_solve_plain_device", C.c_int,
      [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.c_void_p]),
     ("bg_bdf_solve_device", C.c_int,

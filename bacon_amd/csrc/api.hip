@@ -648,6 +648,25 @@ extern "C" int bg_rk_solve_device(int method, int rhs, int semantics, const bg_e
     return rk_launch_ensemble(method, rhs, semantics, *d, lc, (hipStream_t)stream);
 }
 
+extern "C" int bg_rk_autotune_device(int method, int rhs, int semantics,
+                                     const bg_ensemble_desc *d, bg_tuning *out, void *stream) {
+    if (!d || !out || !is_rk(method)) return BG_ERR_INVALID_ARGUMENT;
+    return rk_autotune(method, rhs, semantics, *d, *out, (hipStream_t)stream);
+}
+
+extern "C" int bg_rk_schedule_device(const bg_tuning *tuning, const bg_ensemble_desc *d,
+                                     int32_t *order, void *stream) {
+    if (!d || !tuning) return BG_ERR_INVALID_ARGUMENT;
+    return rk_schedule(*tuning, *d, order, (hipStream_t)stream);
+}
+
+extern "C" int bg_rk_solve_tuned_device(int method, int rhs, int semantics,
+                                        const bg_ensemble_desc *d, const bg_tuning *tuning,
+                                        void *stream) {
+    if (!d || !tuning || !is_rk(method)) return BG_ERR_INVALID_ARGUMENT;
+    return rk_solve_tuned(method, rhs, semantics, *d, *tuning, (hipStream_t)stream);
+}
+
 extern "C" int bg_rk_solve_plain_device(int method, int rhs, int semantics,
                                         const bg_ensemble_desc *d, void *stream) {
     if (!d || !is_rk(method)) return BG_ERR_INVALID_ARGUMENT;

@@ -16,6 +16,14 @@ int rk_launch_plain(int method, int rhs, int semantics, const bg_ensemble_desc &
                     hipStream_t stream);
 bool rk_supported(int method, int rhs, uint32_t dim);
 
+// Autotuner (tune.hip)
+int rk_autotune(int method, int rhs, int semantics, const bg_ensemble_desc &d, bg_tuning &tn,
+                hipStream_t stream);
+int rk_schedule(const bg_tuning &tn, const bg_ensemble_desc &d, int32_t *order,
+                hipStream_t stream);
+int rk_solve_tuned(int method, int rhs, int semantics, const bg_ensemble_desc &d,
+                   const bg_tuning &tn, hipStream_t stream);
+
 // RK45 for y' = A y + b with one shared A (dim 128), the stage right-hand
 // sides as FP64 MFMA GEMMs over tiles of 16 instances (linear_kernels.hip).
 int linear_launch_ensemble(int method, int semantics, const bg_ensemble_desc &d,

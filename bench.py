@@ -103,6 +103,26 @@ def rk_ensemble(cx, w, order=True):
     return b, d
 
 
+def autotune(cx, d, args):
+    """Tune the plan for descriptor d (d.order must be unset)."""
+    L = cx.L
+    tn = L.Tuning()
+    t = time.perf_counter()
+    rc = cx.lib.bg_rk_autotune_device(L.RK45, L.RHS["lorenz"], L.INTENDED, C.byref(d),
+                                      C.byref(tn), cx.sh)
+    if rc != 0:
+        raise RuntimeError(f"bg_rk_autotune_device returned {rc}")
+    tn.pad = 0
+    log(f"[bench] autotune {time.perf_counter() - t:.2f} s: {tn.as_dict()}")
+    if args.block:
+        tn.launch.block = args.block
+    if args.blocks_per_cu:
+        tn.launch.blocks_per_cu = args.blocks_per_cu
+    if args.refill_min:
+        tn.launch.refill_min = args.refill_min
+    return tn
+
+
 def time_launches(cx, fn, steps, warmup):
     torch = cx.torch
     for _ in range(warmup):
@@ -172,13 +192,31 @@ def main():
     n = args.n
     log(f"[bench] generating workload n={n}")
     w = W.lorenz_batched(n)
-    b, d_full = rk_ensemble(cx, w)
-    launch = L.Launch(block=args.block, blocks_per_cu=args.blocks_per_cu,
-                      refill_min=args.refill_min)
+    b, d_full = rk_ensemble(cx, w, order=False)
+    # the autotuner fits the schedule model and picks the launch shape once
+    # (a plan, outside the timed region); every timed step computes the
+    # schedule (cost kernel + radix sort) and solves
+    tn = autotune(cx, d_full, args)
+    # one step = schedule (bg_rk_schedule_device) + solve (bg_rk_solve_device
+    # with that order and the tuned shape), i.e. bg_rk_solve_tuned_device
+    # split in two so an event pair brackets the solve kernel alone
+    order_buf = cx.zeros(n, torch.int32)
+    d_sched = L.EnsembleDesc.from_buffer_copy(d_full)
+    d_sched.order = order_buf.data_ptr()
+    kern_events = []
 
     def run_once():
-        rc = cx.lib.bg_rk_solve_device(L.RK45, L.RHS["lorenz"], L.INTENDED, C.byref(d_full),
-                                       C.byref(launch), cx.sh)
+        rc = cx.lib.bg_rk_schedule_device(C.byref(tn), C.byref(d_full),
+                                          C.c_void_p(order_buf.data_ptr()), cx.sh)
+        if rc != 0:
+            raise RuntimeError(f"bg_rk_schedule_device returned {rc}")
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(cx.stream)
+        rc = cx.lib.bg_rk_solve_device(L.RK45, L.RHS["lorenz"], L.INTENDED, C.byref(d_sched),
+                                       C.byref(tn.launch), cx.sh)
+        e1.record(cx.stream)
+        kern_events.append((e0, e1))
         if rc != 0:
             raise RuntimeError(f"bg_rk_solve_device returned {rc}")
 
@@ -192,9 +230,11 @@ def main():
         del src, dst
 
     log(f"[bench] headline: warmup {args.warmup}, {args.steps} timed steps")
-    wall, kern = time_launches(cx, run_once, args.steps, args.warmup)
+    wall, step_ev = time_launches(cx, run_once, args.steps, args.warmup)
+    kern = float(np.mean([a.elapsed_time(b) for a, b in kern_events[-args.steps:]])) / 1000.0
     ms_per_step = wall * 1000.0
-    log(f"[bench] ms/step {ms_per_step:.3f} (events {kern * 1e3:.3f})")
+    log(f"[bench] ms/step {ms_per_step:.3f} (step events {step_ev * 1e3:.3f}, "
+        f"solve kernel {kern * 1e3:.3f})")
 
     st = b["status"].cpu().numpy()
     cnt = b["counts"].cpu().numpy().astype(np.int64)
@@ -224,7 +264,9 @@ def main():
             "n_instances": n,
             "semantics": "intended",
             "parallelism": "one lane per instance, persistent grid, wave-ballot refill, "
-                           "longest-first schedule",
+                           "longest-first schedule from the autotuner's fitted cost model "
+                           "(computed on device each step), autotuned launch shape",
+            "tuning": tn.as_dict(),
         },
         "accepted_steps_per_s": n_acc / wall,
         "attempts_per_launch": attempts,
@@ -244,6 +286,7 @@ def main():
             "kernel": "rk_ensemble_kernel<RhsLorenz, Tableau<Fehlberg45,false>>",
             "flops_per_attempt": FLOPS_ATTEMPT_LORENZ,
             "avg_kernel_ms": kern * 1e3,
+            "step_events_ms": step_ev * 1e3,
             "algorithmic_hbm_bytes": n * BYTES_PER_INSTANCE,
         },
     }
@@ -263,18 +306,22 @@ def main():
         log("[bench] uniform tol 1e-8 ensemble")
         wu = dict(w)
         wu["tol"] = np.full(n, 1e-8)
-        bu, du = rk_ensemble(cx, wu)
+        bu, du = rk_ensemble(cx, wu, order=False)
+        tnu = autotune(cx, du, args)
 
         def run_u():
-            rc = cx.lib.bg_rk_solve_device(L.RK45, L.RHS["lorenz"], L.INTENDED, C.byref(du),
-                                           C.byref(launch), cx.sh)
+            rc = cx.lib.bg_rk_solve_tuned_device(L.RK45, L.RHS["lorenz"], L.INTENDED,
+                                                 C.byref(du), C.byref(tnu), cx.sh)
             if rc != 0:
                 raise RuntimeError(rc)
         wall_u, kern_u = time_launches(cx, run_u, 2, 1)
         cu = bu["counts"].cpu().numpy().astype(np.int64)
         tol8 = {"solves_per_s": n / wall_u, "ms_per_solve": wall_u * 1e3,
                 "accepted_steps_per_s": int(cu[:, 0].sum()) / wall_u,
-                "fp64_tflops": int(cu.sum()) * FLOPS_ATTEMPT_LORENZ / kern_u / 1e12}
+                "fp64_tflops": int(cu.sum()) * FLOPS_ATTEMPT_LORENZ / kern_u / 1e12,
+                "frac_of_fp64_peak": int(cu.sum()) * FLOPS_ATTEMPT_LORENZ / kern_u / 1e12
+                                     / PEAK_FP64_TFLOPS,
+                "tuning": tnu.as_dict()}
         if not args.no_plain:
             plain_u, pbu = plain_path(cx, wu, 64)
             tol8["plain_gpu_solves_per_s"] = plain_u

@@ -204,6 +204,43 @@ int bg_rk_solve_device(int method, int rhs, int semantics, const bg_ensemble_des
 /* The plain per-instance path over the same descriptor (synchronous). */
 int bg_rk_solve_plain_device(int method, int rhs, int semantics, const bg_ensemble_desc *d,
                              void *stream);
+/* ------------------------------------------------------------------------ */
+/* Autotuner (no counterpart in the reference: its solver runs one instance) */
+/* ------------------------------------------------------------------------ */
+#define BG_TUNE_MAX_PARAMS 8
+/* A tuned plan for one RK workload.
+ * Schedule: instances are handed to lanes longest-first by the predicted cost
+ *   log cost = coef[0] + coef[1] log(t_end - t0) + coef[2] log(tol)
+ *            + sum_k coef[3 + k] params[k]
+ * whose coefficients are a least-squares fit to the step attempts of a solve
+ * of the workload (on a subsample of up to 16,384 instances).
+ * Launch: the fastest of a set of (block size, resident blocks per CU)
+ * shapes, each timed on the whole problem with that schedule. */
+typedef struct {
+    bg_launch launch;
+    uint32_t n_params;   /* parameter terms in the model                     */
+    uint32_t pilot_n;    /* instances the model was fitted on                */
+    double coef[3 + BG_TUNE_MAX_PARAMS];
+    double pilot_ms;     /* the solve in input order, default shape          */
+    double default_ms;   /* default shape with the fitted schedule           */
+    double best_ms;      /* the chosen shape with the fitted schedule        */
+    uint32_t n_trials;
+    uint32_t pad;
+} bg_tuning;
+
+/* Tune `d` (d->order must be NULL).  Runs 1 + n_trials full solves on
+ * `stream`, each of which writes valid results into d's outputs; returns
+ * when they are done. */
+int bg_rk_autotune_device(int method, int rhs, int semantics, const bg_ensemble_desc *d,
+                          bg_tuning *out, void *stream);
+/* The plan's schedule permutation of d's instances into order[n] (device). */
+int bg_rk_schedule_device(const bg_tuning *tuning, const bg_ensemble_desc *d, int32_t *order,
+                          void *stream);
+/* Schedule + solve with the plan's launch shape, asynchronous on `stream`.
+ * If d->order is set it is used as given. */
+int bg_rk_solve_tuned_device(int method, int rhs, int semantics, const bg_ensemble_desc *d,
+                             const bg_tuning *tuning, void *stream);
+
 /* Batched BDF (BDF6 / BDF2) in one persistent launch on `stream`
  * (asynchronous).  lu_bits = 64: Jacobian inverse in FP64; 32: inverse
  * factored in FP32, every residual and the Broyden iteration in FP64. */
