@@ -61,7 +61,7 @@ BYTES_PER_INSTANCE = 8 * 8 + 4 + 8 * 4 + 4 + 8
 # HBM bytes per launch measured with rocprofv3 FETCH_SIZE and WRITE_SIZE (each
 # scaled by a 1 GiB copy-kernel calibration); see
 # profiles/rk_ensemble_lorenz_2p20.md
-MEASURED_TRAFFIC_BYTES = 1.054e9 + 0.238e9
+MEASURED_TRAFFIC_BYTES = 1.065e9 + 0.237e9
 
 
 def log(msg):
@@ -168,6 +168,8 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--extras", default="tol8,adjoint,bdf,linear",
+                    help="comma list of extra measurements: tol8, adjoint, bdf, linear")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline ensemble (used for profiling)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -301,7 +303,8 @@ def main():
                                "identical_to_batched": same}
         result["speedup_vs_plain"] = solves_per_s / plain_sps
 
-    if not args.no_extras:
+    extras = set() if args.no_extras else {x for x in args.extras.split(",") if x}
+    if "tol8" in extras:
         # ---- the metric's tolerance: uniform tol 1e-8 ---------------------------
         log("[bench] uniform tol 1e-8 ensemble")
         wu = dict(w)
@@ -331,6 +334,7 @@ def main():
         result["uniform_tol_1e-8"] = tol8
         del bu, du
 
+    if "adjoint" in extras:
         # ---- adjoint gradients --------------------------------------------------
         from bacon_amd.adjoint import solve_vjp
         na = 262144
@@ -365,6 +369,7 @@ def main():
         }
         del r, res
 
+    if "bdf" in extras:
         # ---- BDF6 Brusselator ---------------------------------------------------
         nb_ = 131072
         log(f"[bench] BDF6, {nb_} Brusselator instances")
@@ -406,6 +411,7 @@ def main():
         result["bdf6_brusselator"] = bdf
         del bb, db
 
+    if "linear" in extras:
         # ---- linear shared-A RK45 (FP64 MFMA tiles) -----------------------------
         nl = 65536
         log(f"[bench] linear shared-A RK45, {nl} instances, N={LIN_N}")
@@ -439,7 +445,11 @@ def main():
                "fp64_tflops": int(cl.sum()) * FLOPS_ATTEMPT_LINEAR / kern_l / 1e12,
                "frac_of_fp64_peak": int(cl.sum()) * FLOPS_ATTEMPT_LINEAR / kern_l / 1e12
                                     / PEAK_FP64_TFLOPS,
-               "max_abs_err_vs_closed_form": float(np.abs(yl - ye).max())}
+               "max_abs_err_vs_closed_form": float(np.abs(yl - ye).max()),
+               # rocprofv3 on linear_rk_kernel, see profiles/linear_mfma_65536.md
+               "mfma_utilisation_rocprof": 0.78,
+               "hbm_traffic_bytes_rocprof": 68.4e6 + 68.6e6,
+               "algorithmic_hbm_bytes": nl * LIN_N * 8 * 2 + nl * 24}
         if not args.no_cpu_baseline:
             import oracle as O
             m, threads = 64, min(os.cpu_count() or 1, 16)
