@@ -53,6 +53,17 @@ class EnsembleDesc(C.Structure):
     ]
 
 
+PATH_STOPS_ONLY = 1
+
+
+class PathDesc(C.Structure):
+    _fields_ = [
+        ("capacity", C.c_uint32), ("every", C.c_uint32), ("n_stops", C.c_uint32),
+        ("flags", C.c_uint32), ("stops", C.c_void_p), ("path_t", C.c_void_p),
+        ("path_y", C.c_void_p), ("path_len", C.c_void_p),
+    ]
+
+
 TUNE_MAX_PARAMS = 8
 
 
@@ -122,6 +133,8 @@ SIGNATURES = [
     ("bg_ensemble_destroy", None, [C.c_void_p]),
     ("bg_rk_solve_device", C.c_int,
      [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.POINTER(Launch), C.c_void_p]),
+    ("bg_rk_solve_path_device", C.c_int,
+     [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.POINTER(PathDesc), C.c_void_p]),
     ("bg_rk_autotune_device", C.c_int,
      [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.POINTER(Tuning), C.c_void_p]),
     ("bg_rk_schedule_device", C.c_int,

@@ -648,6 +648,13 @@ extern "C" int bg_rk_solve_device(int method, int rhs, int semantics, const bg_e
     return rk_launch_ensemble(method, rhs, semantics, *d, lc, (hipStream_t)stream);
 }
 
+extern "C" int bg_rk_solve_path_device(int method, int rhs, int semantics,
+                                       const bg_ensemble_desc *d, const bg_path_desc *path,
+                                       void *stream) {
+    if (!d || !path || !is_rk(method)) return BG_ERR_INVALID_ARGUMENT;
+    return rk_launch_path_ensemble(method, rhs, semantics, *d, *path, (hipStream_t)stream);
+}
+
 extern "C" int bg_rk_autotune_device(int method, int rhs, int semantics,
                                      const bg_ensemble_desc *d, bg_tuning *out, void *stream) {
     if (!d || !out || !is_rk(method)) return BG_ERR_INVALID_ARGUMENT;

@@ -15,6 +15,8 @@ int rk_launch_ensemble(int method, int rhs, int semantics, const bg_ensemble_des
 int rk_launch_plain(int method, int rhs, int semantics, const bg_ensemble_desc &d,
                     hipStream_t stream);
 bool rk_supported(int method, int rhs, uint32_t dim);
+int rk_launch_path_ensemble(int method, int rhs, int semantics, const bg_ensemble_desc &d,
+                            const bg_path_desc &p, hipStream_t stream);
 
 // Autotuner (tune.hip)
 int rk_autotune(int method, int rhs, int semantics, const bg_ensemble_desc &d, bg_tuning &tn,

@@ -182,6 +182,103 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// Path output for an ensemble: the same persistent refill scheme as
+// rk_ensemble_kernel, and each accepted (t, y) the reference's IVPIterator
+// would yield (rk.rs:419, ivp.rs:220-238) is written to a per-instance ring
+// buffer: every `every`-th accepted step, or only landings on the shared stop
+// times and the end.  Stops: the end clamp of rk.rs:366-368 aims at the next
+// stop time and an accepted clamped step lands exactly on it.
+template <class RHS, class Tab>
+__global__ void __launch_bounds__(256)
+    rk_path_ensemble_kernel(const volatile bg_ensemble_desc *dv, const bg_path_desc *__restrict__ pd,
+                            uint32_t *__restrict__ counter) {
+    constexpr int D = RHS::D;
+    r8_init();
+    RkLane<RHS, Tab> L;
+    const int lane = __lane_id();
+    const uint64_t max_att = dv->max_attempts;
+    const uint32_t cap = (uint32_t)((max_att && max_att < kDefaultMaxAttempts) ? max_att
+                                                                                : kDefaultMaxAttempts);
+    const uint32_t every = pd->every ? pd->every : 1u;
+    const uint32_t capacity = pd->capacity;
+    const bool stops_only = (pd->flags & BG_PATH_STOPS_ONLY) != 0;
+    const double *stops = pd->stops;
+    int64_t inst = kNeedWork;
+    uint32_t attempts = 0, nacc = 0, nrej = 0, next_stop = 0, n_stops = 0, nrec = 0;
+    for (;;) {
+        const uint64_t need = __ballot(inst == kNeedWork);
+        if (need != 0) {
+            const int leader = __ffsll((unsigned long long)need) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(need));
+            base = __shfl(base, leader);
+            if (inst == kNeedWork) {
+                const uint64_t pos =
+                    (uint64_t)base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
+                const uint64_t n = dv->n;
+                if (pos < n) {
+                    const int32_t *order = dv->order;
+                    inst = order ? (int64_t)order[pos] : (int64_t)pos;
+                    rk_load(L, desc_inputs(dv), (uint64_t)inst);
+                    attempts = nacc = nrej = next_stop = nrec = 0;
+                    n_stops = 0;
+                    while (n_stops < pd->n_stops && stops[n_stops] < L.end) ++n_stops;
+                    while (next_stop < n_stops && stops[next_stop] <= L.t) ++next_stop;
+                } else {
+                    inst = kExhausted;
+                }
+            }
+        }
+        if (__ballot(inst >= 0) == 0) break;
+        int st = (inst >= 0 && L.t >= L.end) ? BG_OK : -1;
+        for (;;) {
+            if (__ballot(inst >= 0 && st >= 0) != 0) break;
+            if (inst >= 0 && st < 0) {
+                if (attempts >= cap) {
+                    st = BG_MAXIMUM_ITERATIONS_EXCEEDED;
+                } else {
+                    const bool to_stop = next_stop < n_stops;
+                    const double target = to_stop ? stops[next_stop] : L.end;
+                    bool clamped;
+                    const int r = L.attempt_to(target, clamped);
+                    ++attempts;
+                    if (r == STEP_ACCEPTED) {
+                        ++nacc;
+                        bool at_stop = false;
+                        if (clamped && to_stop) {
+                            L.t = target;
+                            ++next_stop;
+                            at_stop = true;
+                        }
+                        const bool last = L.t >= L.end;
+                        const bool rec = stops_only ? (at_stop || last) : (nacc % every == 0);
+                        if (rec) {
+                            const uint64_t n = dv->n, i = (uint64_t)inst;
+                            const uint64_t e = capacity ? (uint64_t)(nrec % capacity) : 0;
+                            if (capacity) {
+                                pd->path_t[e * n + i] = L.t;
+#pragma unroll
+                                for (int c = 0; c < D; ++c) pd->path_y[(e * D + c) * n + i] = L.y[c];
+                            }
+                            ++nrec;
+                        }
+                        if (last) st = BG_OK;
+                    } else if (r == STEP_REJECTED) {
+                        ++nrej;
+                    } else {
+                        st = r;
+                    }
+                }
+            }
+        }
+        if (inst >= 0 && st >= 0) {
+            rk_store(L, desc_outputs(dv), (uint64_t)inst, st, nacc, nrej);
+            if (pd->path_len) pd->path_len[inst] = nrec;
+            inst = kNeedWork;
+        }
+    }
+}
+
 // Plain path: one instance per launch, one lane.
 template <class RHS, class Tab>
 __global__ void __launch_bounds__(64) rk_plain_kernel(const bg_ensemble_desc d, uint64_t k) {
@@ -382,6 +479,49 @@ int rk_launch_path(int method, int rhs, int semantics, uint32_t dim, PathState *
         using T = typename decltype(ttag)::type;
         rk_path_kernel<R, T><<<1, 64, 0, stream>>>(state, path_t, path_y, cap);
         BG_HIP_CHECK(hipGetLastError());
+        return BG_OK;
+    });
+}
+
+__global__ void rk_path_setup_kernel(uint32_t *counter, bg_ensemble_desc *dst,
+                                     const bg_ensemble_desc d, bg_path_desc *pdst,
+                                     const bg_path_desc p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *counter = 0u;
+        *dst = d;
+        *pdst = p;
+    }
+}
+
+int rk_launch_path_ensemble(int method, int rhs, int semantics, const bg_ensemble_desc &d,
+                            const bg_path_desc &p, hipStream_t stream) {
+    return visit_rk(method, rhs, semantics, d.dim, [&](auto rtag, auto ttag) -> int {
+        using R = typename decltype(rtag)::type;
+        using T = typename decltype(ttag)::type;
+        int rc = check_desc(d, R::P);
+        if (rc != BG_OK || d.n == 0) return rc;
+        if (p.capacity && (!p.path_t || !p.path_y)) return BG_ERR_INVALID_ARGUMENT;
+        if (p.n_stops && !p.stops) return BG_ERR_INVALID_ARGUMENT;
+        int dev = 0;
+        BG_HIP_CHECK(hipGetDevice(&dev));
+        int per_cu = 0;
+        BG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, rk_path_ensemble_kernel<R, T>, 256, 0));
+        if (per_cu < 1) per_cu = 1;
+        uint64_t grid = (uint64_t)per_cu * (uint64_t)device_cu_count(dev);
+        const uint64_t need = (d.n + 255) / 256;
+        if (grid > need) grid = need;
+        // workspace: [0,256) counter, then the two descriptor copies
+        const size_t off_p = (256 + sizeof(bg_ensemble_desc) + 255) & ~(size_t)255;
+        char *ws = nullptr;
+        BG_HIP_CHECK(hipMallocAsync((void **)&ws, off_p + sizeof(bg_path_desc), stream));
+        uint32_t *counter = (uint32_t *)ws;
+        bg_ensemble_desc *ddesc = (bg_ensemble_desc *)(ws + 256);
+        bg_path_desc *pdesc = (bg_path_desc *)(ws + off_p);
+        rk_path_setup_kernel<<<1, 64, 0, stream>>>(counter, ddesc, d, pdesc, p);
+        rk_path_ensemble_kernel<R, T><<<(uint32_t)grid, 256, 0, stream>>>(ddesc, pdesc, counter);
+        BG_HIP_CHECK(hipGetLastError());
+        BG_HIP_CHECK(hipFreeAsync(ws, stream));
         return BG_OK;
     });
 }

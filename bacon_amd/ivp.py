@@ -316,3 +316,55 @@ def solve_ensemble(method, w: dict, semantics=L.INTENDED, plain=False, max_attem
     if lu_bits != 64:
         e.with_lu_precision(lu_bits)
     return e.solve(plain=plain)
+
+
+def solve_path(method, w: dict, capacity: int, every: int = 1, stops=None,
+               stops_only: bool = False, semantics=L.INTENDED, device: str = "cuda:0"):
+    """Ensemble solve with path output (bg_rk_solve_path_device) for a
+    workload dict (see bacon_amd.workloads).  Returns numpy arrays: final t,
+    y, status, counts, and the per-instance ring buffers path_t[capacity][n],
+    path_y[capacity][dim][n], path_len[n]."""
+    import torch
+    n, dim = w["y0"].shape[1], w["dim"]
+
+    def T(a, dt=torch.float64):
+        return torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)
+
+    def arr(v):
+        return np.array(np.broadcast_to(np.asarray(v, dtype=np.float64), (n,)))
+
+    keep = dict(y0=T(w["y0"]), params=T(w["params"] if w["params"].size else np.zeros((1, n))),
+                t0=T(arr(w.get("t0", 0.0))), t_end=T(arr(w["t_end"])), tol=T(arr(w["tol"])),
+                shared=T(w["shared"]) if w.get("shared") is not None else None,
+                t=torch.zeros(n, dtype=torch.float64, device=device),
+                y=torch.zeros((dim, n), dtype=torch.float64, device=device),
+                st=torch.full((n,), -1, dtype=torch.int32, device=device),
+                cnt=torch.zeros((n, 2), dtype=torch.int32, device=device),
+                pt=torch.zeros((max(capacity, 1), n), dtype=torch.float64, device=device),
+                py=torch.zeros((max(capacity, 1), dim, n), dtype=torch.float64, device=device),
+                pl=torch.zeros(n, dtype=torch.int32, device=device))
+    st_arr = None
+    if stops is not None and len(stops):
+        st_arr = T(np.asarray(stops, dtype=np.float64))
+    d = L.EnsembleDesc(
+        n=n, dim=dim, n_params=w["params"].shape[0], y0=keep["y0"].data_ptr(),
+        params=keep["params"].data_ptr(),
+        shared=keep["shared"].data_ptr() if keep["shared"] is not None else None,
+        t0=keep["t0"].data_ptr(), t_end=keep["t_end"].data_ptr(), tol=keep["tol"].data_ptr(),
+        dt_min_s=w["dt_min"], dt_max_s=w["dt_max"],
+        t_out=keep["t"].data_ptr(), y_out=keep["y"].data_ptr(), status=keep["st"].data_ptr(),
+        counts=keep["cnt"].data_ptr())
+    p = L.PathDesc(capacity=capacity, every=every,
+                   n_stops=0 if st_arr is None else st_arr.numel(),
+                   flags=L.PATH_STOPS_ONLY if stops_only else 0,
+                   stops=None if st_arr is None else st_arr.data_ptr(),
+                   path_t=keep["pt"].data_ptr(), path_y=keep["py"].data_ptr(),
+                   path_len=keep["pl"].data_ptr())
+    _check(L.lib().bg_rk_solve_path_device(method, _rhs_id(w["rhs"]), semantics, C.byref(d),
+                                           C.byref(p), None))
+    torch.cuda.synchronize()
+    cnt = keep["cnt"].cpu().numpy().astype(np.int64)
+    return dict(t=keep["t"].cpu().numpy(), y=keep["y"].cpu().numpy(),
+                status=keep["st"].cpu().numpy(), n_accepted=cnt[:, 0], n_rejected=cnt[:, 1],
+                path_t=keep["pt"].cpu().numpy()[:capacity], path_y=keep["py"].cpu().numpy()[:capacity],
+                path_len=keep["pl"].cpu().numpy().astype(np.int64))

@@ -204,6 +204,27 @@ int bg_rk_solve_device(int method, int rhs, int semantics, const bg_ensemble_des
 /* The plain per-instance path over the same descriptor (synchronous). */
 int bg_rk_solve_plain_device(int method, int rhs, int semantics, const bg_ensemble_desc *d,
                              void *stream);
+/* Path output for an ensemble: every accepted (t, y) that the reference's
+ * IVPIterator yields (rk.rs:419, ivp.rs:220-238), per instance, into a ring
+ * buffer of `capacity` entries (entry k of an instance at k % capacity;
+ * path_len[i] counts all entries, so > capacity means the ring wrapped).
+ * Optional shared stop times: the end clamp (rk.rs:366-368) aims at the next
+ * stop and an accepted clamped step lands exactly on it. */
+#define BG_PATH_STOPS_ONLY 1u   /* record only landings on stops and the end */
+typedef struct {
+    uint32_t capacity;     /* entries per instance (0: count only)             */
+    uint32_t every;        /* record every k-th accepted step (0 or 1: each)   */
+    uint32_t n_stops;
+    uint32_t flags;        /* BG_PATH_STOPS_ONLY                               */
+    const double *stops;   /* [n_stops] increasing times shared by all, or NULL */
+    double *path_t;        /* [capacity][n]                                    */
+    double *path_y;        /* [capacity][dim][n]                               */
+    uint32_t *path_len;    /* [n] entries recorded, or NULL                    */
+} bg_path_desc;
+/* Batched RK with path output, one persistent launch, asynchronous. */
+int bg_rk_solve_path_device(int method, int rhs, int semantics, const bg_ensemble_desc *d,
+                            const bg_path_desc *path, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Autotuner (no counterpart in the reference: its solver runs one instance) */
 /* ------------------------------------------------------------------------ */
