@@ -73,8 +73,10 @@ struct RhsBrusselatorG {
         const double dx = 1.0 / CELLS;
         const double cl = alpha / (dx * dx);
         const double uuv = u * u * v;
-        const double lap = left - 2.0 * y + right;
-        f = isu ? A + uuv - (B + 1.0) * u + cl * lap : B * u - uuv + cl * lap;
+        const double lap = fma(-2.0, y, left) + right;
+        // u: A + u^2 v - (B + 1) u + cl lap;  v: B u - u^2 v + cl lap
+        const double react = isu ? fma(-(B + 1.0), u, A + uuv) : fma(B, u, -uuv);
+        f = fma(cl, lap, react);
         return true;
     }
     // FLOPs of one full evaluation (all 2*CELLS components)
@@ -130,7 +132,7 @@ struct BdfLane {
     static constexpr int D = RHS::D, W = D;
     using C = BdfCoef<O>;
     Grp<W> grp;
-    double *lds;  // this group's W x (W+1) transpose buffer
+    double *lds;  // this group's W x (W+1) transpose buffer, then 2 scalars
     // instance scalars (replicated across the group)
     double t, dt, end, tol, dt_min, dt_max;
     int yield_memory;
@@ -197,8 +199,10 @@ struct BdfLane {
         constexpr bool own = LOWER && Sem::lower_own_hist;
         g = (-fx) * dt * (LOWER ? C::lo(0) : C::hi(0));
 #pragma unroll
-        for (int ind = 1; ind < O; ++ind)
-            g += hist[O - ind] * (own ? C::lo(ind) : C::hi(ind));
+        for (int ind = 1; ind < O; ++ind) {
+            const double c = own ? C::lo(ind) : C::hi(ind);
+            if (c != 0.0) g = fma(hist[O - ind], c, g);
+        }
         g = g + x;
         return true;
     }
@@ -209,7 +213,7 @@ struct BdfLane {
         __builtin_amdgcn_wave_barrier();
         double acc = 0.0;
 #pragma unroll
-        for (int j = 0; j < W; ++j) acc += jinv[j] * lds[j];
+        for (int j = 0; j < W; ++j) acc = fma(jinv[j], lds[j], acc);
         __builtin_amdgcn_wave_barrier();
         return -acc;
     }
@@ -258,7 +262,7 @@ struct BdfLane {
                 const LuT f = b[k];
 #pragma unroll
                 for (int j = 0; j < W; ++j)
-                    if (j != k) b[j] = b[j] - f * (LuT)lds[j];
+                    if (j != k) b[j] = fma(-f, (LuT)lds[j], b[j]);
                 b[k] = -f * (LuT)lds[k];
             }
             __builtin_amdgcn_wave_barrier();
@@ -325,7 +329,7 @@ struct BdfLane {
             __builtin_amdgcn_wave_barrier();
             const double w = (s + adj) / pp;
 #pragma unroll
-            for (int j = 0; j < W; ++j) jinv[j] += lds[j] * w;
+            for (int j = 0; j < W; ++j) jinv[j] = fma(lds[j], w, jinv[j]);
             __builtin_amdgcn_wave_barrier();
             s = neg_jinv_times(F);
             guess += s;
@@ -445,19 +449,18 @@ struct BdfLane {
 // ---------------------------------------------------------------------------
 template <class RHS, int O, class Sem, class LuT>
 __global__ void __launch_bounds__(64)
-    bdf_ensemble_kernel(const bg_ensemble_desc *__restrict__ dp, uint32_t *__restrict__ counter) {
+    bdf_ensemble_kernel(const volatile bg_ensemble_desc *dv, uint32_t *__restrict__ counter) {
     constexpr int W = RHS::D;
     constexpr int GPW = 64 / W;  // groups per wave
-    __shared__ double lds_all[GPW][W * (W + 1)];
-    const bg_ensemble_desc d = *dp;
+    __shared__ double lds_all[GPW][W * (W + 1) + 2];
     const int lane = __lane_id();
     const int grp_id = lane / W;
     BdfLane<RHS, O, Sem, LuT> L;
     L.grp.g = lane % W;
     L.lds = lds_all[grp_id];
+    const uint64_t max_att = dv->max_attempts;
     const uint32_t cap =
-        (uint32_t)((d.max_attempts && d.max_attempts < kBdfDefaultMaxSteps) ? d.max_attempts
-                                                                          : kBdfDefaultMaxSteps);
+        (uint32_t)((max_att && max_att < kBdfDefaultMaxSteps) ? max_att : kBdfDefaultMaxSteps);
     int64_t inst = -1;  // -1 need work, -2 exhausted
     for (;;) {
         // group leaders that need work
@@ -473,9 +476,10 @@ __global__ void __launch_bounds__(64)
             const bool my_need = (need >> (grp_id * W)) & 1ull;
             if (my_need) {
                 const uint64_t k = (uint64_t)base + (uint64_t)__popcll(below);
-                if (k < d.n) {
-                    inst = d.order ? (int64_t)d.order[k] : (int64_t)k;
-                    L.load(d, (uint64_t)inst);
+                if (k < dv->n) {
+                    const int32_t *order = dv->order;
+                    inst = order ? (int64_t)order[k] : (int64_t)k;
+                    L.load(desc_inputs(dv), (uint64_t)inst);
                 } else {
                     inst = -2;
                 }
@@ -505,6 +509,7 @@ __global__ void __launch_bounds__(64)
                     break;
                 }
             }
+            const bg_ensemble_desc d = desc_outputs(dv);
             const uint64_t n = d.n, i = (uint64_t)inst;
             if (d.y_out) d.y_out[(uint64_t)L.grp.g * n + i] = L.y;
             if (L.grp.g == 0) {
@@ -535,7 +540,7 @@ __global__ void __launch_bounds__(64)
     bdf_path_kernel(BdfPathState *__restrict__ st, double *__restrict__ path_t,
                     double *__restrict__ path_y, uint32_t cap) {
     constexpr int W = RHS::D;
-    __shared__ double lds[W * (W + 1)];
+    __shared__ double lds[W * (W + 1) + 2];
     const int lane = __lane_id();
     if (lane >= W) return;
     BdfLane<RHS, O, Sem, LuT> L;

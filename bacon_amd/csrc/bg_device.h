@@ -360,4 +360,36 @@ struct RkLane {
     }
 };
 
+// Descriptor reads through a volatile pointer to a device-memory copy: each
+// field is a fresh load at the point of use, so the persistent kernel does not
+// keep ~30 descriptor SGPRs live across the attempt loop.
+__device__ __forceinline__ bg_ensemble_desc desc_inputs(const volatile bg_ensemble_desc *dv) {
+    bg_ensemble_desc d;
+    d.n = dv->n;
+    d.y0 = dv->y0;
+    d.params = dv->params;
+    d.t0 = dv->t0;
+    d.t_end = dv->t_end;
+    d.tol = dv->tol;
+    d.dt_min = dv->dt_min;
+    d.dt_max = dv->dt_max;
+    d.t0_s = dv->t0_s;
+    d.t_end_s = dv->t_end_s;
+    d.tol_s = dv->tol_s;
+    d.dt_min_s = dv->dt_min_s;
+    d.dt_max_s = dv->dt_max_s;
+    return d;
+}
+
+__device__ __forceinline__ bg_ensemble_desc desc_outputs(const volatile bg_ensemble_desc *dv) {
+    bg_ensemble_desc d;
+    d.n = dv->n;
+    d.t_out = dv->t_out;
+    d.y_out = dv->y_out;
+    d.status = dv->status;
+    d.counts = dv->counts;
+    d.work = dv->work;
+    return d;
+}
+
 }  // namespace bg
