@@ -127,11 +127,19 @@ enum : int { BDF_YIELD = -1, BDF_REDO = -2, BDF_DONE = -3 };
 // ---------------------------------------------------------------------------
 // one group's BDF solver state (per lane: its component / its row)
 // ---------------------------------------------------------------------------
-template <class RHS, int O, class Sem, class LuT>
+// SPLIT = 1: a group of W = D lanes per instance; lane g owns component g and
+// row g of the Jacobian inverse.  SPLIT = 2: the whole wave per instance
+// (2 W = 64 lanes): lane (h, g) = h W + g owns component g (both halves hold
+// the same component values, computed redundantly) and the half row
+// Jinv[g][h W/2 .. h W/2 + W/2 - 1].  The split halves the registers for the
+// inverse, runs the FD Jacobian's +dt and -dt evaluations on the two halves
+// at once, and leaves no second instance in the wave to diverge from.
+template <class RHS, int O, class Sem, class LuT, int SPLIT = 1>
 struct BdfLane {
-    static constexpr int D = RHS::D, W = D;
+    static constexpr int D = RHS::D, W = D, WJ = W / SPLIT;
     using C = BdfCoef<O>;
     Grp<W> grp;
+    int h = 0;    // half of the wave (SPLIT = 2)
     double *lds;  // this group's W x (W+1) transpose buffer, then 2 scalars
     // instance scalars (replicated across the group)
     double t, dt, end, tol, dt_min, dt_max;
@@ -141,7 +149,7 @@ struct BdfLane {
     // per-lane component data
     double y, save;
     double hist[O];  // prev_values front..back (own component)
-    double jinv[W];  // row g of the Jacobian inverse
+    double jinv[WJ];  // row g of the Jacobian inverse (SPLIT = 2: half row h)
     double p[RHS::P > 0 ? RHS::P : 1];
     // counters
     uint32_t n_newton, n_lu;
@@ -213,7 +221,8 @@ struct BdfLane {
         __builtin_amdgcn_wave_barrier();
         double acc = 0.0;
 #pragma unroll
-        for (int j = 0; j < W; ++j) acc = fma(jinv[j], lds[j], acc);
+        for (int j = 0; j < WJ; ++j) acc = fma(jinv[j], lds[h * WJ + j], acc);
+        if constexpr (SPLIT == 2) acc += __shfl_xor(acc, W);
         __builtin_amdgcn_wave_barrier();
         return -acc;
     }
@@ -225,6 +234,74 @@ struct BdfLane {
     // its columns permuted by col(), which one LDS scatter/gather undoes.
     // Arithmetic in LuT; returns false on an exact zero (or NaN) pivot.
     __device__ __forceinline__ bool invert() {
+        if constexpr (SPLIT == 2) return invert_split();
+        else return invert_group();
+    }
+
+    // SPLIT = 2: the same exchange steps on half rows.  Column k lives in
+    // half k / WJ; the other half fetches a(g, k) across the halves.
+    __device__ __forceinline__ bool invert_split() {
+        LuT b[WJ];
+#pragma unroll
+        for (int j = 0; j < WJ; ++j) b[j] = (LuT)lds[grp.g * (W + 1) + h * WJ + j];
+        __builtin_amdgcn_wave_barrier();
+        bool used = false, ok = true;
+        int my_col = 0;
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            const int hk = k / WJ, kk = k % WJ;
+            const LuT mk = b[kk];
+            const LuT other = (LuT)__shfl_xor((double)mk, W);
+            const LuT agk = (h == hk) ? mk : other;
+            double best = used ? -1.0 : fabs((double)agk);
+            int who = grp.g;
+#pragma unroll
+            for (int m = W / 2; m >= 1; m >>= 1) {
+                const double ob = __shfl_xor(best, m, W);
+                const int ow = __shfl_xor(who, m, W);
+                if (ob > best || (ob == best && ow < who)) {
+                    best = ob;
+                    who = ow;
+                }
+            }
+            if (!(best > 0.0)) ok = false;
+            const bool piv = (who == grp.g);
+            if (piv) {
+                used = true;
+                my_col = k;
+                const LuT ip = (LuT)1 / agk;
+#pragma unroll
+                for (int j = 0; j < WJ; ++j) b[j] *= ip;
+                if (h == hk) b[kk] = ip;
+#pragma unroll
+                for (int j = 0; j < WJ; ++j) lds[h * WJ + j] = (double)b[j];
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (!piv) {
+                const LuT f = agk;
+#pragma unroll
+                for (int j = 0; j < WJ; ++j) b[j] = fma(-f, (LuT)lds[h * WJ + j], b[j]);
+                // column k itself becomes -f / pivot (the published row holds
+                // 1 / pivot there)
+                if (h == hk) b[kk] = -f * (LuT)lds[k];
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+#pragma unroll
+        for (int c = 0; c < WJ; ++c) lds[my_col * (W + 1) + h * WJ + c] = (double)b[c];
+        lds[grp.g * (W + 1) + W] = (double)my_col;
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int q = 0; q < WJ; ++q) {
+            int cq = (int)lds[(h * WJ + q) * (W + 1) + W];
+            cq = cq < 0 ? 0 : (cq >= W ? W - 1 : cq);
+            jinv[q] = lds[grp.g * (W + 1) + cq];
+        }
+        __builtin_amdgcn_wave_barrier();
+        return ok;
+    }
+
+    __device__ __forceinline__ bool invert_group() {
         LuT b[W];
 #pragma unroll
         for (int j = 0; j < W; ++j) b[j] = (LuT)lds[grp.g * (W + 1) + j];
@@ -297,13 +374,25 @@ struct BdfLane {
 #pragma unroll 1
         for (int c = 0; c < W; ++c) {
             const bool mine = (grp.g == c);
-            if (mine) guess += dt;
-            double above, below;
-            if (!residual<LOWER>(tt, guess, above)) return BG_USER_ERROR;
-            if (mine) guess -= 2.0 * dt;
-            if (!residual<LOWER>(tt, guess, below)) return BG_USER_ERROR;
-            if (mine) guess += dt;
-            lds[grp.g * (W + 1) + c] = (above + Sem::fd_sign * below) * denom;
+            if constexpr (SPLIT == 2) {
+                // half 0 evaluates at x + dt e_c, half 1 at (x + dt e_c) - 2 dt e_c
+                const double xp = guess + dt, xm = xp - 2.0 * dt;
+                const double xg = mine ? (h == 0 ? xp : xm) : guess;
+                double r;
+                if (!residual<LOWER>(tt, xg, r)) return BG_USER_ERROR;
+                const double other = __shfl_xor(r, W);
+                const double above = h == 0 ? r : other, below = h == 0 ? other : r;
+                if (mine) guess = xm + dt;  // the reference's restore (bdf.rs:405)
+                lds[grp.g * (W + 1) + c] = (above + Sem::fd_sign * below) * denom;
+            } else {
+                if (mine) guess += dt;
+                double above, below;
+                if (!residual<LOWER>(tt, guess, above)) return BG_USER_ERROR;
+                if (mine) guess -= 2.0 * dt;
+                if (!residual<LOWER>(tt, guess, below)) return BG_USER_ERROR;
+                if (mine) guess += dt;
+                lds[grp.g * (W + 1) + c] = (above + Sem::fd_sign * below) * denom;
+            }
         }
         __builtin_amdgcn_wave_barrier();
         ++n_lu;
@@ -319,17 +408,24 @@ struct BdfLane {
             // p = (-s)^T adj ; u = s^T Jinv (column sums through LDS)
             const double pp = grp.sum(adj * (-s));
 #pragma unroll
-            for (int j = 0; j < W; ++j) lds[grp.g * (W + 1) + j] = jinv[j] * s;
+            for (int j = 0; j < WJ; ++j) lds[grp.g * (W + 1) + h * WJ + j] = jinv[j] * s;
             __builtin_amdgcn_wave_barrier();
             double u = 0.0;
+            if constexpr (SPLIT == 2) {
+                // column g: rows of half h, then the other half's partial sum
 #pragma unroll
-            for (int k = 0; k < W; ++k) u += lds[k * (W + 1) + grp.g];
+                for (int k = 0; k < WJ; ++k) u += lds[(h * WJ + k) * (W + 1) + grp.g];
+                u += __shfl_xor(u, W);
+            } else {
+#pragma unroll
+                for (int k = 0; k < W; ++k) u += lds[k * (W + 1) + grp.g];
+            }
             __builtin_amdgcn_wave_barrier();
             lds[grp.g] = u;
             __builtin_amdgcn_wave_barrier();
             const double w = (s + adj) / pp;
 #pragma unroll
-            for (int j = 0; j < W; ++j) jinv[j] = fma(lds[j], w, jinv[j]);
+            for (int j = 0; j < WJ; ++j) jinv[j] = fma(lds[h * WJ + j], w, jinv[j]);
             __builtin_amdgcn_wave_barrier();
             s = neg_jinv_times(F);
             guess += s;
@@ -447,16 +543,18 @@ struct BdfLane {
 // ---------------------------------------------------------------------------
 // persistent ensemble kernel: one instance per group, group refill
 // ---------------------------------------------------------------------------
-template <class RHS, int O, class Sem, class LuT>
+template <class RHS, int O, class Sem, class LuT, int SPLIT>
 __global__ void __launch_bounds__(64)
     bdf_ensemble_kernel(const volatile bg_ensemble_desc *dv, uint32_t *__restrict__ counter) {
     constexpr int W = RHS::D;
-    constexpr int GPW = 64 / W;  // groups per wave
+    constexpr int GW = W * SPLIT;  // lanes per instance
+    constexpr int GPW = 64 / GW;   // groups per wave
     __shared__ double lds_all[GPW][W * (W + 1) + 2];
     const int lane = __lane_id();
-    const int grp_id = lane / W;
-    BdfLane<RHS, O, Sem, LuT> L;
+    const int grp_id = lane / GW;
+    BdfLane<RHS, O, Sem, LuT, SPLIT> L;
     L.grp.g = lane % W;
+    L.h = (lane % GW) / W;
     L.lds = lds_all[grp_id];
     const uint64_t max_att = dv->max_attempts;
     const uint32_t cap =
@@ -464,7 +562,7 @@ __global__ void __launch_bounds__(64)
     int64_t inst = -1;  // -1 need work, -2 exhausted
     for (;;) {
         // group leaders that need work
-        const bool leader = (L.grp.g == 0);
+        const bool leader = (lane % GW == 0);
         const uint64_t need = __ballot(leader && inst == -1);
         if (need != 0) {
             const int first = __ffsll((unsigned long long)need) - 1;
@@ -472,8 +570,8 @@ __global__ void __launch_bounds__(64)
             if (lane == first) base = atomicAdd(counter, (uint32_t)__popcll(need));
             base = __shfl(base, first);
             // rank of my group's leader among needing leaders
-            const uint64_t below = need & ((1ull << (grp_id * W)) - 1ull);
-            const bool my_need = (need >> (grp_id * W)) & 1ull;
+            const uint64_t below = need & ((1ull << (grp_id * GW)) - 1ull);
+            const bool my_need = (need >> (grp_id * GW)) & 1ull;
             if (my_need) {
                 const uint64_t k = (uint64_t)base + (uint64_t)__popcll(below);
                 if (k < dv->n) {
@@ -511,8 +609,8 @@ __global__ void __launch_bounds__(64)
             }
             const bg_ensemble_desc d = desc_outputs(dv);
             const uint64_t n = d.n, i = (uint64_t)inst;
-            if (d.y_out) d.y_out[(uint64_t)L.grp.g * n + i] = L.y;
-            if (L.grp.g == 0) {
+            if (d.y_out && L.h == 0) d.y_out[(uint64_t)L.grp.g * n + i] = L.y;
+            if (leader) {
                 if (d.t_out) d.t_out[i] = L.t;
                 if (d.status) d.status[i] = st;
                 if (d.counts) {
@@ -687,12 +785,14 @@ int bdf_launch_ensemble(int method, int rhs, int semantics, int lu_fp32, const b
                          if (R::P > 0 && (!d.params || d.n_params < (uint32_t)R::P))
                              return BG_ERR_INVALID_ARGUMENT;
                          constexpr int W = R::D;
-                         constexpr int GPW = 64 / W;
+                         // a 32-component instance takes the whole wave
+                         constexpr int SP = W == 32 ? 2 : 1;
+                         constexpr int GPW = 64 / (W * SP);
                          int dev = 0;
                          BG_HIP_CHECK(hipGetDevice(&dev));
                          int per_cu = 0;
                          BG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                             &per_cu, bdf_ensemble_kernel<R, O, S, F>, 64, 0));
+                             &per_cu, bdf_ensemble_kernel<R, O, S, F, SP>, 64, 0));
                          if (per_cu < 1) per_cu = 1;
                          uint64_t grid = (uint64_t)per_cu * (uint64_t)device_cu_count(dev);
                          const uint64_t need = (d.n + GPW - 1) / GPW;
@@ -703,7 +803,7 @@ int bdf_launch_ensemble(int method, int rhs, int semantics, int lu_fp32, const b
                          uint32_t *counter = (uint32_t *)ws;
                          bg_ensemble_desc *ddesc = (bg_ensemble_desc *)(ws + 256);
                          bdf_setup_kernel<<<1, 64, 0, stream>>>(counter, ddesc, d);
-                         bdf_ensemble_kernel<R, O, S, F>
+                         bdf_ensemble_kernel<R, O, S, F, SP>
                              <<<(uint32_t)grid, 64, 0, stream>>>(ddesc, counter);
                          BG_HIP_CHECK(hipGetLastError());
                          BG_HIP_CHECK(hipFreeAsync(ws, stream));
