@@ -23,7 +23,42 @@ namespace bg {
 static constexpr uint32_t kBdfDefaultMaxSteps = 1000000u;
 
 // ---------------------------------------------------------------------------
-// group collectives
+// cross-lane moves without LDS traffic (gfx950 lane layouts, measured):
+//   DPP row_ror:1 (0x121): lane i <- lane (i - 1) mod 16 within its row of 16
+//   DPP row_ror:15 (0x12F): lane i <- lane (i + 1) mod 16
+//   DPP quad_perm [1,0,3,2] (0xB1): i <- i ^ 1;  [2,3,0,1] (0x4E): i <- i ^ 2
+//   DPP row_half_mirror (0x141): i <- 7 - i within 8; row_mirror (0x140):
+//     i <- 15 - i within 16
+//   ds_swizzle bitmask 0x401F: i <- i ^ 16 within 32
+//   v_permlane32_swap(v, v): [0] = v[l mod 32], [1] = v[32 + l mod 32]
+// ---------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)b, CTRL, 0xF, 0xF, false);
+    const uint32_t hi =
+        (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double xor16_f64(double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_swizzle((int)(uint32_t)b, 0x401F);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_swizzle((int)(uint32_t)(b >> 32), 0x401F);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+// value of lane l ^ 32
+__device__ __forceinline__ double xor32_f64(double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)b, (uint32_t)b, false, false);
+    const auto hi =
+        __builtin_amdgcn_permlane32_swap((uint32_t)(b >> 32), (uint32_t)(b >> 32), false, false);
+    const bool low = __lane_id() < 32;
+    const uint32_t l = low ? lo[1] : lo[0], h = low ? hi[1] : hi[0];
+    return __builtin_bit_cast(double, ((uint64_t)h << 32) | l);
+}
+
+// ---------------------------------------------------------------------------
+// group collectives (every lane of the group active)
 // ---------------------------------------------------------------------------
 template <int W>
 struct Grp {
@@ -32,12 +67,54 @@ struct Grp {
         if constexpr (W == 1) return v;
         return __shfl(v, src, W);
     }
+    // all-reduce sum; symmetric pairings, so every lane gets the same bits
     __device__ __forceinline__ double sum(double v) const {
+        if constexpr (W == 32) {
+            v += dpp_f64<0xB1>(v);
+            v += dpp_f64<0x4E>(v);
+            v += dpp_f64<0x141>(v);
+            v += dpp_f64<0x140>(v);
+            v += xor16_f64(v);
+            return v;
+        } else {
 #pragma unroll
-        for (int m = W / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, W);
-        return v;
+            for (int m = W / 2; m >= 1; m >>= 1) v += __shfl_xor(v, m, W);
+            return v;
+        }
+    }
+    // all-reduce maximum (IEEE maxNum: a NaN operand loses)
+    __device__ __forceinline__ double max(double v) const {
+        if constexpr (W == 32) {
+            v = fmax(v, dpp_f64<0xB1>(v));
+            v = fmax(v, dpp_f64<0x4E>(v));
+            v = fmax(v, dpp_f64<0x141>(v));
+            v = fmax(v, dpp_f64<0x140>(v));
+            v = fmax(v, xor16_f64(v));
+            return v;
+        } else {
+#pragma unroll
+            for (int m = W / 2; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, W));
+            return v;
+        }
     }
 };
+
+// Pivot choice key: |a| with its 5 lowest mantissa bits replaced by 31 - g,
+// so the maximum key names the row with the largest |a| and the lowest lane
+// among equal ones (ties below 2^-47 relative resolve towards lower lanes).
+// Rows not eligible (already pivots, NaN) get key 0.
+__device__ __forceinline__ double pivot_key(double a, bool eligible, int g) {
+    const double m = fabs(a);
+    if (!eligible || !(m == m)) return 0.0;
+    const uint64_t b = (__builtin_bit_cast(uint64_t, m) & ~(uint64_t)31) | (uint64_t)(31 - g);
+    return __builtin_bit_cast(double, b);
+}
+__device__ __forceinline__ int pivot_lane(double key) {
+    return 31 - (int)(__builtin_bit_cast(uint64_t, key) & 31);
+}
+__device__ __forceinline__ bool pivot_nonzero(double key) {
+    return (__builtin_bit_cast(uint64_t, key) & ~(uint64_t)31) != 0;
+}
 
 // ---------------------------------------------------------------------------
 // right-hand sides evaluated cooperatively: lane g returns f_g(t, y)
@@ -65,9 +142,18 @@ struct RhsBrusselatorG {
         const bool isu = g < CELLS;
         const int c = isu ? g : g - CELLS;
         const int base = isu ? 0 : CELLS;
-        const double left = grp.bcast(y, base + (c + CELLS - 1) % CELLS);
-        const double right = grp.bcast(y, base + (c + 1) % CELLS);
-        const double other = grp.bcast(y, isu ? g + CELLS : g - CELLS);
+        double left, right, other;
+        if constexpr (CELLS == 16) {
+            // a species' 16 cells fill one DPP row: periodic neighbours are
+            // row rotations, the other species is the lane 16 apart
+            left = dpp_f64<0x121>(y);
+            right = dpp_f64<0x12F>(y);
+            other = xor16_f64(y);
+        } else {
+            left = grp.bcast(y, base + (c + CELLS - 1) % CELLS);
+            right = grp.bcast(y, base + (c + 1) % CELLS);
+            other = grp.bcast(y, isu ? g + CELLS : g - CELLS);
+        }
         const double u = isu ? y : other, v = isu ? other : y;
         const double A = p[0], B = p[1], alpha = p[2];
         const double dx = 1.0 / CELLS;
@@ -222,17 +308,18 @@ struct BdfLane {
         double acc = 0.0;
 #pragma unroll
         for (int j = 0; j < WJ; ++j) acc = fma(jinv[j], lds[h * WJ + j], acc);
-        if constexpr (SPLIT == 2) acc += __shfl_xor(acc, W);
+        if constexpr (SPLIT == 2) acc += xor32_f64(acc);
         __builtin_amdgcn_wave_barrier();
         return -acc;
     }
 
     // In-place Gauss-Jordan inversion (exchange steps) of the group matrix
-    // whose row g is staged in row g of the LDS buffer, with partial pivoting on rows that are not swapped:
-    // column k pivots on the unused row p with the largest |a(p,k)| (lowest
-    // lane on ties).  Afterwards lane p holds row col(p) of the inverse with
-    // its columns permuted by col(), which one LDS scatter/gather undoes.
-    // Arithmetic in LuT; returns false on an exact zero (or NaN) pivot.
+    // whose row g is staged in row g of the LDS buffer, with partial pivoting
+    // on rows that are not swapped: column k pivots on the unused row p with
+    // the largest |a(p,k)| (pivot_key: lowest lane on ties).  Afterwards lane
+    // p holds row col(p) of the inverse with its columns permuted by col(),
+    // which one LDS scatter/gather undoes.  Arithmetic in LuT; returns false
+    // on a zero (or NaN) pivot.
     __device__ __forceinline__ bool invert() {
         if constexpr (SPLIT == 2) return invert_split();
         else return invert_group();
@@ -251,21 +338,11 @@ struct BdfLane {
         for (int k = 0; k < W; ++k) {
             const int hk = k / WJ, kk = k % WJ;
             const LuT mk = b[kk];
-            const LuT other = (LuT)__shfl_xor((double)mk, W);
+            const LuT other = (LuT)xor32_f64((double)mk);
             const LuT agk = (h == hk) ? mk : other;
-            double best = used ? -1.0 : fabs((double)agk);
-            int who = grp.g;
-#pragma unroll
-            for (int m = W / 2; m >= 1; m >>= 1) {
-                const double ob = __shfl_xor(best, m, W);
-                const int ow = __shfl_xor(who, m, W);
-                if (ob > best || (ob == best && ow < who)) {
-                    best = ob;
-                    who = ow;
-                }
-            }
-            if (!(best > 0.0)) ok = false;
-            const bool piv = (who == grp.g);
+            const double key = grp.max(pivot_key((double)agk, !used, grp.g));
+            if (!pivot_nonzero(key)) ok = false;
+            const bool piv = (pivot_lane(key) == grp.g);
             if (piv) {
                 used = true;
                 my_col = k;
@@ -310,19 +387,9 @@ struct BdfLane {
         int my_col = 0;
 #pragma unroll
         for (int k = 0; k < W; ++k) {
-            double best = used ? -1.0 : fabs((double)b[k]);
-            int who = grp.g;
-#pragma unroll
-            for (int m = W / 2; m >= 1; m >>= 1) {
-                const double ob = __shfl_xor(best, m, W);
-                const int ow = __shfl_xor(who, m, W);
-                if (ob > best || (ob == best && ow < who)) {
-                    best = ob;
-                    who = ow;
-                }
-            }
-            if (!(best > 0.0)) ok = false;
-            const bool piv = (who == grp.g);
+            const double key = grp.max(pivot_key((double)b[k], !used, grp.g));
+            if (!pivot_nonzero(key)) ok = false;
+            const bool piv = (pivot_lane(key) == grp.g);
             if (piv) {
                 used = true;
                 my_col = k;
@@ -380,7 +447,7 @@ struct BdfLane {
                 const double xg = mine ? (h == 0 ? xp : xm) : guess;
                 double r;
                 if (!residual<LOWER>(tt, xg, r)) return BG_USER_ERROR;
-                const double other = __shfl_xor(r, W);
+                const double other = xor32_f64(r);
                 const double above = h == 0 ? r : other, below = h == 0 ? other : r;
                 if (mine) guess = xm + dt;  // the reference's restore (bdf.rs:405)
                 lds[grp.g * (W + 1) + c] = (above + Sem::fd_sign * below) * denom;
@@ -415,7 +482,7 @@ struct BdfLane {
                 // column g: rows of half h, then the other half's partial sum
 #pragma unroll
                 for (int k = 0; k < WJ; ++k) u += lds[(h * WJ + k) * (W + 1) + grp.g];
-                u += __shfl_xor(u, W);
+                u += xor32_f64(u);
             } else {
 #pragma unroll
                 for (int k = 0; k < W; ++k) u += lds[k * (W + 1) + grp.g];
