@@ -141,6 +141,13 @@ SIGNATURES = [
      [C.POINTER(Tuning), C.POINTER(EnsembleDesc), C.c_void_p, C.c_void_p]),
     ("bg_rk_solve_tuned_device", C.c_int,
      [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.POINTER(Tuning), C.c_void_p]),
+    ("bg_graph_capture_rk", C.c_int,
+     [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.POINTER(Launch),
+      C.POINTER(C.c_void_p)]),
+    ("bg_graph_capture_bdf", C.c_int,
+     [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.POINTER(C.c_void_p)]),
+    ("bg_graph_launch", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("bg_graph_destroy", None, [C.c_void_p]),
     ("bg_rk_solve_plain_device", C.c_int,
      [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.c_void_p]),
     ("bg_bdf_solve_device", C.c_int,

@@ -843,7 +843,7 @@ bool bdf_supported(int method, int rhs, uint32_t dim) {
 }
 
 int bdf_launch_ensemble(int method, int rhs, int semantics, int lu_fp32, const bg_ensemble_desc &d,
-                        hipStream_t stream) {
+                        hipStream_t stream, void *ws_in) {
     return visit_bdf(method, rhs, semantics, d.dim, lu_fp32,
                      [&](auto rtag, auto otag, auto stag, auto ftag) -> int {
                          using R = typename decltype(rtag)::type;
@@ -867,16 +867,17 @@ int bdf_launch_ensemble(int method, int rhs, int semantics, int lu_fp32, const b
                          uint64_t grid = (uint64_t)per_cu * (uint64_t)device_cu_count(dev);
                          const uint64_t need = (d.n + GPW - 1) / GPW;
                          if (grid > need) grid = need;
-                         char *ws = nullptr;
-                         BG_HIP_CHECK(
-                             hipMallocAsync((void **)&ws, 256 + sizeof(bg_ensemble_desc), stream));
+                         char *ws = (char *)ws_in;
+                         if (!ws_in)
+                             BG_HIP_CHECK(
+                                 hipMallocAsync((void **)&ws, kLaunchWorkspaceBytes, stream));
                          uint32_t *counter = (uint32_t *)ws;
                          bg_ensemble_desc *ddesc = (bg_ensemble_desc *)(ws + 256);
                          bdf_setup_kernel<<<1, 64, 0, stream>>>(counter, ddesc, d);
                          bdf_ensemble_kernel<R, O, S, F, SP>
                              <<<(uint32_t)grid, 64, 0, stream>>>(ddesc, counter);
                          BG_HIP_CHECK(hipGetLastError());
-                         BG_HIP_CHECK(hipFreeAsync(ws, stream));
+                         if (!ws_in) BG_HIP_CHECK(hipFreeAsync(ws, stream));
                          return BG_OK;
                      });
 }

@@ -9,9 +9,15 @@
 
 namespace bg {
 
+// Device workspace of one persistent launch: the work counter at [0, 256),
+// the descriptor copy after it.  A launcher given ws == nullptr takes it from
+// the stream-ordered allocator and frees it on the stream; a caller-owned
+// workspace (graph capture) must hold kLaunchWorkspaceBytes.
+constexpr size_t kLaunchWorkspaceBytes = 256 + ((sizeof(bg_ensemble_desc) + 255) & ~(size_t)255);
+
 // Launches of the RK family.  Each returns a BG_* status.
 int rk_launch_ensemble(int method, int rhs, int semantics, const bg_ensemble_desc &d,
-                       const bg_launch &launch, hipStream_t stream);
+                       const bg_launch &launch, hipStream_t stream, void *ws = nullptr);
 int rk_launch_plain(int method, int rhs, int semantics, const bg_ensemble_desc &d,
                     hipStream_t stream);
 bool rk_supported(int method, int rhs, uint32_t dim);
@@ -29,7 +35,7 @@ int rk_solve_tuned(int method, int rhs, int semantics, const bg_ensemble_desc &d
 // RK45 for y' = A y + b with one shared A (dim 128), the stage right-hand
 // sides as FP64 MFMA GEMMs over tiles of 16 instances (linear_kernels.hip).
 int linear_launch_ensemble(int method, int semantics, const bg_ensemble_desc &d,
-                           hipStream_t stream);
+                           hipStream_t stream, void *ws = nullptr);
 
 // Single-instance path production (one lane).  The state block lives in
 // device memory and is resumable: the kernel stops when the path buffer is
@@ -63,7 +69,7 @@ struct BdfPathState {
 };
 bool bdf_supported(int method, int rhs, uint32_t dim);
 int bdf_launch_ensemble(int method, int rhs, int semantics, int lu_fp32, const bg_ensemble_desc &d,
-                        hipStream_t stream);
+                        hipStream_t stream, void *ws = nullptr);
 int bdf_launch_path(int method, int rhs, int semantics, uint32_t dim, BdfPathState *state,
                     double *path_t, double *path_y, uint32_t cap, hipStream_t stream);
 

@@ -250,7 +250,7 @@ __global__ void linear_setup_kernel(uint32_t *counter, bg_ensemble_desc *dst,
 }
 
 int linear_launch_ensemble(int method, int semantics, const bg_ensemble_desc &d,
-                           hipStream_t stream) {
+                           hipStream_t stream, void *ws_in) {
     if (method != BG_RK45) return BG_ERR_UNSUPPORTED;
     if (d.dim != (uint32_t)LN) return BG_ERR_UNSUPPORTED;
     if (d.n == 0) return BG_OK;
@@ -264,14 +264,14 @@ int linear_launch_ensemble(int method, int semantics, const bg_ensemble_desc &d,
         uint64_t grid = (uint64_t)per_cu * (uint64_t)device_cu_count(dev);
         const uint64_t tiles = (d.n + LT - 1) / LT;
         if (grid > tiles) grid = tiles;
-        char *ws = nullptr;
-        BG_HIP_CHECK(hipMallocAsync((void **)&ws, 256 + sizeof(bg_ensemble_desc), stream));
+        char *ws = (char *)ws_in;
+        if (!ws_in) BG_HIP_CHECK(hipMallocAsync((void **)&ws, kLaunchWorkspaceBytes, stream));
         uint32_t *counter = (uint32_t *)ws;
         bg_ensemble_desc *ddesc = (bg_ensemble_desc *)(ws + 256);
         linear_setup_kernel<<<1, 64, 0, stream>>>(counter, ddesc, d);
         kern<<<(uint32_t)grid, 256, 0, stream>>>(ddesc, counter);
         BG_HIP_CHECK(hipGetLastError());
-        BG_HIP_CHECK(hipFreeAsync(ws, stream));
+        if (!ws_in) BG_HIP_CHECK(hipFreeAsync(ws, stream));
         return BG_OK;
     };
     if (semantics == BG_AS_BUILT) return run(linear_rk_kernel<Tableau<Fehlberg45, true>>);

@@ -390,9 +390,9 @@ static int check_desc(const bg_ensemble_desc &d, int P) {
 }
 
 int rk_launch_ensemble(int method, int rhs, int semantics, const bg_ensemble_desc &d,
-                       const bg_launch &lc, hipStream_t stream) {
+                       const bg_launch &lc, hipStream_t stream, void *ws_in) {
     // the shared-matrix linear right-hand side runs as a tiled MFMA GEMM kernel
-    if (rhs == BG_RHS_LINEAR) return linear_launch_ensemble(method, semantics, d, stream);
+    if (rhs == BG_RHS_LINEAR) return linear_launch_ensemble(method, semantics, d, stream, ws_in);
     return visit_rk(method, rhs, semantics, d.dim, [&](auto rtag, auto ttag) -> int {
         using R = typename decltype(rtag)::type;
         using T = typename decltype(ttag)::type;
@@ -410,9 +410,9 @@ int rk_launch_ensemble(int method, int rhs, int semantics, const bg_ensemble_des
         uint64_t grid = (uint64_t)per_cu * (uint64_t)device_cu_count(dev);
         const uint64_t need = (d.n + block - 1) / block;
         if (grid > need) grid = need;
-        // workspace: [0,256) work counter, [256, 256+sizeof desc) descriptor copy
-        char *ws = nullptr;
-        BG_HIP_CHECK(hipMallocAsync((void **)&ws, 256 + sizeof(bg_ensemble_desc), stream));
+        // workspace: [0,256) work counter, [256, ...) descriptor copy
+        char *ws = (char *)ws_in;
+        if (!ws_in) BG_HIP_CHECK(hipMallocAsync((void **)&ws, kLaunchWorkspaceBytes, stream));
         uint32_t *counter = (uint32_t *)ws;
         bg_ensemble_desc *ddesc = (bg_ensemble_desc *)(ws + 256);
         rk_setup_kernel<<<1, 64, 0, stream>>>(counter, ddesc, d);
@@ -420,7 +420,7 @@ int rk_launch_ensemble(int method, int rhs, int semantics, const bg_ensemble_des
         rk_ensemble_kernel<R, T>
             <<<(uint32_t)grid, block, 0, stream>>>(ddesc, counter, refill_min, lc.prio_cutoff);
         BG_HIP_CHECK(hipGetLastError());
-        BG_HIP_CHECK(hipFreeAsync(ws, stream));
+        if (!ws_in) BG_HIP_CHECK(hipFreeAsync(ws, stream));
         return BG_OK;
     });
 }

@@ -262,6 +262,23 @@ int bg_rk_schedule_device(const bg_tuning *tuning, const bg_ensemble_desc *d, in
 int bg_rk_solve_tuned_device(int method, int rhs, int semantics, const bg_ensemble_desc *d,
                              const bg_tuning *tuning, void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* HIP graph capture (no counterpart in the reference)                      */
+/* ------------------------------------------------------------------------ */
+/* One batched solve captured once and replayed: the setup kernel (counter
+ * reset, descriptor copy) and the persistent solve kernel, whose BDF Newton
+ * iterations run inside it.  The launch workspace is allocated at capture and
+ * owned by the graph.  A replay reads the descriptor's device buffers as they
+ * hold at replay time.  Replays of one graph must not overlap each other. */
+typedef struct bg_graph bg_graph;
+int bg_graph_capture_rk(int method, int rhs, int semantics, const bg_ensemble_desc *d,
+                        const bg_launch *launch, bg_graph **out);
+int bg_graph_capture_bdf(int method, int rhs, int semantics, int lu_bits,
+                         const bg_ensemble_desc *d, bg_graph **out);
+/* Replay on `stream` (hipStream_t, NULL = default), asynchronous. */
+int bg_graph_launch(bg_graph *g, void *stream);
+void bg_graph_destroy(bg_graph *g);
+
 /* Batched BDF (BDF6 / BDF2) in one persistent launch on `stream`
  * (asynchronous).  lu_bits = 64: Jacobian inverse in FP64; 32: inverse
  * factored in FP32, every residual and the Broyden iteration in FP64. */
