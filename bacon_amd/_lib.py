@@ -86,7 +86,7 @@ class Tuning(C.Structure):
 class AdjointDesc(C.Structure):
     _fields_ = [
         ("base", EnsembleDesc), ("n_obs", C.c_uint32), ("checkpoint_every", C.c_uint32),
-        ("max_checkpoints", C.c_uint32), ("pad", C.c_uint32),
+        ("max_checkpoints", C.c_uint32), ("blocks_per_cu", C.c_uint32),
         ("obs_t", C.c_void_p), ("targets", C.c_void_p), ("loss", C.c_void_p),
         ("grad_params", C.c_void_p), ("grad_y0", C.c_void_p), ("obs_y", C.c_void_p),
     ]

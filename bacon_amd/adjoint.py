@@ -24,9 +24,11 @@ def _ptr(t):
 def solve_vjp(rhs: str, y0: torch.Tensor, params: torch.Tensor, obs_t: torch.Tensor,
               targets: torch.Tensor, t_end, tol, dt_min: float = 1e-12, dt_max: float = 0.1,
               t0: float = 0.0, checkpoint_every: int = 16, max_checkpoints: int = 128,
-              stream=None):
+              stream=None, order=None, blocks_per_cu: int = 0):
     """All tensors float64 on the GPU: y0 [D, n], params [P, n], obs_t [n_obs],
-    targets [n_obs, D, n]; t_end / tol scalars or [n] tensors."""
+    targets [n_obs, D, n]; t_end / tol scalars or [n] tensors.  `order` (int32
+    [n], optional) is the forward pass's schedule, e.g. from
+    bg_rk_schedule_device with a plan tuned on the same problem."""
     dev = y0.device
     D, n = y0.shape
     P = params.shape[0]
@@ -46,9 +48,11 @@ def solve_vjp(rhs: str, y0: torch.Tensor, params: torch.Tensor, obs_t: torch.Ten
         t0_s=t0, t_end=_ptr(te), t_end_s=0.0 if te is not None else float(t_end),
         tol=_ptr(tl), tol_s=0.0 if tl is not None else float(tol),
         dt_min_s=dt_min, dt_max_s=dt_max, t_out=_ptr(out["t"]), y_out=_ptr(out["y"]),
-        status=_ptr(out["status"]), counts=_ptr(out["counts"]))
+        status=_ptr(out["status"]), counts=_ptr(out["counts"]),
+        order=_ptr(order) if order is not None else None)
     d = L.AdjointDesc(base=base, n_obs=obs_t.numel(), checkpoint_every=checkpoint_every,
-                      max_checkpoints=max_checkpoints, obs_t=_ptr(obs_t),
+                      max_checkpoints=max_checkpoints, blocks_per_cu=blocks_per_cu,
+                      obs_t=_ptr(obs_t),
                       targets=_ptr(targets), loss=_ptr(out["loss"]),
                       grad_params=_ptr(out["grad_params"]), grad_y0=_ptr(out["grad_y0"]),
                       obs_y=_ptr(out["obs_y"]))

@@ -21,6 +21,8 @@
 //    hand-written Jacobian-vector products.
 // Storage layouts are structure-of-arrays with the instance (or lane slot) as
 // the fastest index, so each lane's loads and stores coalesce with its wave.
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "bg_device.h"
 #include "bg_internal.h"
 
@@ -35,7 +37,18 @@ struct AdjWs {
     double *seg;     // [K][F][slots] backward segment records
     uint64_t slots;
     uint32_t *counters;  // [0] forward queue, [1] backward queue
+    const int32_t *order_b;  // [n] backward schedule: most checkpoints first
 };
+
+// Backward schedule keys: the forward's checkpoint count per instance is the
+// exact number of segments the backward pass will recompute and reverse.
+__global__ void adj_order_keys_kernel(const uint32_t *__restrict__ nck, uint64_t n,
+                                      uint32_t *__restrict__ keys, int32_t *__restrict__ idx) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = nck[i];
+    idx[i] = (int32_t)i;
+}
 
 // Number of observation times strictly before `end` (they become stops).
 __device__ __forceinline__ uint32_t count_stops(const double *obs, uint32_t n_obs, double end) {
@@ -70,8 +83,8 @@ __global__ void __launch_bounds__(256)
             if (inst == -1) {
                 const uint64_t k = (uint64_t)base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
                 if (k < n) {
-                    inst = (int64_t)k;
-                    const uint64_t i = k;
+                    inst = d.order ? (int64_t)d.order[k] : (int64_t)k;
+                    const uint64_t i = (uint64_t)inst;
 #pragma unroll
                     for (int c = 0; c < D; ++c) L.y[c] = d.y0[c * n + i];
 #pragma unroll
@@ -262,8 +275,8 @@ __global__ void __launch_bounds__(256)
             if (inst == -1) {
                 const uint64_t k = (uint64_t)base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
                 if (k < n) {
-                    inst = (int64_t)k;
-                    const uint64_t i = k;
+                    inst = (int64_t)ws.order_b[k];
+                    const uint64_t i = (uint64_t)inst;
 #pragma unroll
                     for (int c = 0; c < RHS::P; ++c) L.p[c] = d.params[c * n + i];
                     L.end = d.t_end ? d.t_end[i] : d.t_end_s;
@@ -406,15 +419,26 @@ int rk_launch_adjoint(int method, int rhs, const bg_adjoint_desc &ad, hipStream_
             &per_cu_b, rk_adjoint_backward_kernel<R, T>, 256, 0));
         per_cu_f = per_cu_f < 1 ? 1 : per_cu_f;
         per_cu_b = per_cu_b < 1 ? 1 : per_cu_b;
+        // fewer resident lanes than the occupancy limit: each lane works
+        // through several instances in schedule order, so lanes finish
+        // together (measured best at 2 blocks per CU for both passes)
+        const int want = ad.blocks_per_cu ? (int)ad.blocks_per_cu : 2;
+        if (per_cu_f > want) per_cu_f = want;
+        if (per_cu_b > want) per_cu_b = want;
         uint64_t grid_f = (uint64_t)per_cu_f * cus, grid_b = (uint64_t)per_cu_b * cus;
         const uint64_t need = (n + 255) / 256;
         if (grid_f > need) grid_f = need;
         if (grid_b > need) grid_b = need;
         const uint64_t slots = grid_b * 256;
         const uint64_t n_res = (uint64_t)(ad.n_obs ? ad.n_obs : 1) * D * n;
+        size_t sort_bytes = 0;
+        BG_HIP_CHECK(rocprim::radix_sort_pairs_desc(
+            nullptr, sort_bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+            (const int32_t *)nullptr, (int32_t *)nullptr, (uint32_t)n, 0, 32, stream));
         const size_t bytes = 256 + sizeof(bg_adjoint_desc) + 64 +
                              sizeof(double) * (maxck * F * n + n_res + K * F * slots) +
-                             sizeof(uint32_t) * n + 256;
+                             sizeof(uint32_t) * n + 256 + 4 * (sizeof(uint32_t) * n + 256) +
+                             sort_bytes;
         char *buf = nullptr;
         BG_HIP_CHECK(hipMallocAsync((void **)&buf, bytes, stream));
         AdjWs ws;
@@ -427,8 +451,23 @@ int rk_launch_adjoint(int method, int rhs, const bg_adjoint_desc &ad, hipStream_
         ws.seg = ws.res + n_res;
         ws.nck = (uint32_t *)(ws.seg + K * F * slots);
         ws.slots = slots;
+        auto align = [](char *q) { return q + (256 - ((uintptr_t)q % 256)) % 256; };
+        char *q = align((char *)(ws.nck + n));
+        uint32_t *keys = (uint32_t *)q;
+        q = align(q + sizeof(uint32_t) * n);
+        uint32_t *keys_sorted = (uint32_t *)q;
+        q = align(q + sizeof(uint32_t) * n);
+        int32_t *idx = (int32_t *)q;
+        q = align(q + sizeof(uint32_t) * n);
+        int32_t *order_b = (int32_t *)q;
+        q = align(q + sizeof(uint32_t) * n);
+        void *sort_tmp = q;
+        ws.order_b = order_b;
         adj_setup_kernel<<<1, 64, 0, stream>>>(ws.counters, dd, ad);
         rk_adjoint_forward_kernel<R, T><<<(uint32_t)grid_f, 256, 0, stream>>>(dd, ws);
+        adj_order_keys_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, stream>>>(ws.nck, n, keys, idx);
+        BG_HIP_CHECK(rocprim::radix_sort_pairs_desc(sort_tmp, sort_bytes, keys, keys_sorted, idx,
+                                                    order_b, (uint32_t)n, 0, 32, stream));
         rk_adjoint_backward_kernel<R, T><<<(uint32_t)grid_b, 256, 0, stream>>>(dd, ws);
         BG_HIP_CHECK(hipGetLastError());
         BG_HIP_CHECK(hipFreeAsync(buf, stream));
