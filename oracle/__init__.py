@@ -33,7 +33,7 @@ DYNAMIC_ON_STATIC = 11
 STATIC_ON_DYNAMIC = 12
 INVALID_ARGUMENT = -1
 
-RK45, RK23, BDF6, BDF2 = 0, 1, 2, 3
+RK45, RK23, BDF6, BDF2, ADAMS5, ADAMS3 = 0, 1, 2, 3, 4, 5
 AS_BUILT, INTENDED = 0, 1
 PROFILE_BACON, PROFILE_DEVICE = 0, 1
 
@@ -109,6 +109,9 @@ def lib():
         L.or_lu_inverse.argtypes = [C.c_int32, _dp, _dp]
         L.or_rk_tableau.argtypes = [C.c_int32, C.c_int32, _dp, _dp, _dp, _dp]
         L.or_bdf_coefficients.argtypes = [C.c_int32, _dp, _dp]
+        L.or_adams_coefficients.argtypes = [C.c_int32, _dp, _dp, _dp]
+        L.or_rootn.argtypes = [C.c_double, C.c_int32]
+        L.or_rootn.restype = C.c_double
         L.or_rk_replay.argtypes = [C.POINTER(Problem), C.c_int64, _dp, _dp, _dp]
         for name in ("or_builder_new", "or_builder_new_dyn"):
             getattr(L, name).argtypes = [C.POINTER(Builder), C.c_int32, C.c_int32]
@@ -235,6 +238,16 @@ def bdf_coefficients(method=BDF6):
     hi = np.zeros(7); lo = np.zeros(7)
     o = lib().or_bdf_coefficients(method, _ptr(hi), _ptr(lo))
     return hi[:o], lo[:o]
+
+
+def adams_coefficients(method=ADAMS5):
+    pred = np.zeros(5); corr = np.zeros(5); err = np.zeros(1)
+    o = lib().or_adams_coefficients(method, _ptr(pred), _ptr(corr), _ptr(err))
+    return pred[:o], corr[:o], float(err[0])
+
+
+def rootn(x, n):
+    return lib().or_rootn(float(x), n)
 
 
 def rk_replay(rhs, y0, t_seq, dt_seq, params=None, method=RK45,

@@ -1115,6 +1115,339 @@ static int bdf_solve(const or_problem *p, double *y_out, or_result *r, or_path *
 }
 
 /* ------------------------------------------------------------------ */
+/* Adams-Bashforth-Moulton predictor-corrector (adams.rs:336-633)     */
+/* ------------------------------------------------------------------ */
+
+/* x^(1/n) for n = 3 or 5 from exact operations only (the DEVICE profile's
+ * replacement for powf(1/order), adams.rs:516,544): x = m 2^e with m in
+ * [1, 2); m^(1/n) by a degree-6 polynomial in (m - 1.5) interpolating at
+ * Chebyshev nodes (relative error < 2.5e-7); 2^(e/n) = 2^floor(e/n) times the
+ * rounded constant 2^((e mod n)/n).  0 -> 0, inf -> inf, NaN -> NaN. */
+static double dev_rootn(double x, int n) {
+    static const double C3[7] = {1.1447142425533323, 0.254383453864114, -0.05653050145457822,
+                                 0.02085714929540036, -0.009260767947395939,
+                                 0.0051666994182390956, -0.0027033338703399646};
+    static const double C5[7] = {1.0844717711976986, 0.14459836703782772,
+                                 -0.038560210159460984, 0.015356114792636277,
+                                 -0.007159250830014004, 0.004170036250587015,
+                                 -0.002242595967870246};
+    static const double S3[3] = {1.0, 1.2599210498948732, 1.5874010519681994};
+    static const double S5[5] = {1.0, 1.148698354997035, 1.3195079107728942, 1.515716566510398,
+                                 1.7411011265922482};
+    if (!(x == x) || x == 0.0 || x == INFINITY) return x;
+    const double *c = n == 3 ? C3 : C5;
+    const double *sc = n == 3 ? S3 : S5;
+    int ex = 0;
+    const double m = frexp(x, &ex);
+    const double u = fma(m, 2.0, -1.5);
+    const int e = ex - 1;
+    double p = c[6];
+    for (int k = 5; k >= 0; --k) p = fma(p, u, c[k]);
+    int a = e / n, b = e % n;
+    if (b < 0) {
+        b += n;
+        a -= 1;
+    }
+    return ldexp(p * sc[b], a);
+}
+
+#define ACAP 16
+typedef struct {
+    const or_problem *p;
+    int O, D, prof, intended;
+    double pred[5], corr[5], err_coef;
+    double time, dt, end, tol, dt_min, dt_max;
+    double two, half, one_sixth, one_tenth, four, order;
+    double state[MAXD], save_state[MAXD], implicit[MAXD];
+    /* prev_values (t, y) and prev_derivatives deques */
+    double vt[ACAP], vy[ACAP][MAXD];
+    int vhead, vlen;
+    double dv[ACAP][MAXD];
+    int dhead, dlen;
+    int yield_memory;
+    int64_t n_rhs, n_reject;
+} adams_solver;
+
+static double *adams_v(adams_solver *s, int k) { return s->vy[(s->vhead + k) % ACAP]; }
+static double adams_vt(adams_solver *s, int k) { return s->vt[(s->vhead + k) % ACAP]; }
+static double *adams_d(adams_solver *s, int k) { return s->dv[(s->dhead + k) % ACAP]; }
+static void adams_push_v(adams_solver *s, double t, const double *y) {
+    const int slot = (s->vhead + s->vlen) % ACAP;
+    s->vt[slot] = t;
+    memcpy(s->vy[slot], y, sizeof(double) * (size_t)s->D);
+    s->vlen++;
+}
+static void adams_push_d(adams_solver *s, const double *f) {
+    const int slot = (s->dhead + s->dlen) % ACAP;
+    memcpy(s->dv[slot], f, sizeof(double) * (size_t)s->D);
+    s->dlen++;
+}
+static void adams_pop_v(adams_solver *s) {
+    if (s->vlen) {
+        s->vhead = (s->vhead + 1) % ACAP;
+        s->vlen--;
+    }
+}
+static void adams_pop_d(adams_solver *s) {
+    if (s->dlen) {
+        s->dhead = (s->dhead + 1) % ACAP;
+        s->dlen--;
+    }
+}
+
+static int adams_rhs(adams_solver *s, double t, const double *y, double *f) {
+    s->n_rhs++;
+    int rc = or_rhs_eval(s->p->rhs, s->prof, s->D, t, y, s->p->params, s->p->shared, f);
+    if (rc == OR_INVALID_ARGUMENT) return rc;
+    return rc == OR_OK ? OR_OK : OR_USER_ERROR;
+}
+
+/* adams.rs:344-396: classical RK4; states (and their derivatives) after
+ * steps 1..n-1 and the final one are pushed onto the deques */
+static int adams_runge_kutta(adams_solver *s, int iterations) {
+    const int D = s->D, pr = s->prof;
+    double k1[MAXD], k2[MAXD], k3[MAXD], k4[MAXD], mid[MAXD], f[MAXD];
+    for (int it = 0; it < iterations; ++it) {
+        int rc = adams_rhs(s, s->time, s->state, k1);
+        if (rc) return rc;
+        for (int d = 0; d < D; ++d) k1[d] *= s->dt;
+        for (int d = 0; d < D; ++d) mid[d] = madd(pr, k1[d], s->half, s->state[d]);
+        rc = adams_rhs(s, madd(pr, s->half, s->dt, s->time), mid, k2);
+        if (rc) return rc;
+        for (int d = 0; d < D; ++d) k2[d] *= s->dt;
+        for (int d = 0; d < D; ++d) mid[d] = madd(pr, k2[d], s->half, s->state[d]);
+        rc = adams_rhs(s, madd(pr, s->half, s->dt, s->time), mid, k3);
+        if (rc) return rc;
+        for (int d = 0; d < D; ++d) k3[d] *= s->dt;
+        for (int d = 0; d < D; ++d) mid[d] = s->state[d] + k3[d];
+        rc = adams_rhs(s, s->time + s->dt, mid, k4);
+        if (rc) return rc;
+        for (int d = 0; d < D; ++d) k4[d] *= s->dt;
+        if (it != 0) {
+            rc = adams_rhs(s, s->time, s->state, f);
+            if (rc) return rc;
+            adams_push_d(s, f);
+            adams_push_v(s, s->time, s->state);
+        }
+        for (int d = 0; d < D; ++d) {
+            double sum = madd(pr, k2[d], s->two, k1[d]);
+            sum = madd(pr, k3[d], s->two, sum);
+            sum = sum + k4[d];
+            s->state[d] = madd(pr, sum, s->one_sixth, s->state[d]);
+        }
+        s->time += s->dt;
+    }
+    int rc = adams_rhs(s, s->time, s->state, f);
+    if (rc) return rc;
+    adams_push_d(s, f);
+    adams_push_v(s, s->time, s->state);
+    return OR_OK;
+}
+
+static int adams_init(adams_solver *s, const or_problem *p) {
+    memset(s, 0, sizeof *s);
+    s->p = p;
+    s->O = p->method == OR_ADAMS5 ? 5 : 3;
+    s->D = p->dim;
+    s->prof = p->profile;
+    s->intended = p->semantics == OR_INTENDED;
+    if (s->O == 5) { /* adams.rs:573-600 */
+        const double tf = 24.0, sh = 720.0;
+        s->pred[0] = 55.0 / tf; s->pred[1] = -59.0 / tf; s->pred[2] = 37.0 / tf;
+        s->pred[3] = -9.0 / tf; s->pred[4] = 0.0;
+        s->corr[0] = 251.0 / sh; s->corr[1] = 646.0 / sh; s->corr[2] = -264.0 / sh;
+        s->corr[3] = 106.0 / sh; s->corr[4] = -19.0 / sh;
+    } else { /* adams.rs:640-661 */
+        s->pred[0] = 1.0 + 1.0 / 2.0; s->pred[1] = -(1.0 / 2.0); s->pred[2] = 0.0;
+        s->corr[0] = 5.0 / 12.0; s->corr[1] = 2.0 / 3.0; s->corr[2] = -(1.0 / 12.0);
+    }
+    s->err_coef = 19.0 / 270.0;
+    s->two = 2.0;
+    s->half = 1.0 / s->two;
+    s->one_sixth = 1.0 / 6.0;
+    s->one_tenth = 1.0 / 10.0;
+    s->four = s->two * s->two;
+    s->order = (double)s->O;
+    s->dt_max = p->dt_max;
+    s->dt_min = p->dt_min;
+    s->time = p->t0;
+    s->end = p->t_end;
+    s->tol = p->tol;
+    s->dt = (p->dt_max + p->dt_min) * s->half; /* adams.rs:288 */
+    memcpy(s->state, p->y0, sizeof(double) * (size_t)p->dim);
+    return OR_OK;
+}
+
+/* q = (tol / (2 error))^(1/order) (adams.rs:516, 544) */
+static double adams_q(adams_solver *s, double error) {
+    const double r = s->tol / (s->two * error);
+    return s->prof == OR_PROFILE_DEVICE ? dev_rootn(r, s->O) : pow(r, 1.0 / s->order);
+}
+
+/* One AdamsSolver::step (adams.rs:411-561). */
+static int adams_step(adams_solver *s, double *yt, double *yv) {
+    const int O = s->O, D = s->D, pr = s->prof;
+    if (s->yield_memory > 0 && s->yield_memory < O) { /* adams.rs:415-428 */
+        const int item = O - s->yield_memory - 1;
+        s->yield_memory -= 1;
+        if (s->yield_memory == 0) s->yield_memory = O + 1;
+        *yt = adams_vt(s, item);
+        memcpy(yv, adams_v(s, item), sizeof(double) * (size_t)D);
+        return OR_STEP_YIELD;
+    }
+    if (s->yield_memory == O + 1) { /* adams.rs:434-441 */
+        s->yield_memory = 0;
+        adams_push_v(s, s->time, s->state);
+        adams_pop_v(s);
+        if (s->intended) {
+            /* the step's derivative joins the history too; as built the
+             * derivative deque keeps the bootstrap's (one step stale) */
+            adams_push_d(s, s->implicit);
+            adams_pop_d(s);
+        }
+        *yt = s->time;
+        memcpy(yv, s->state, sizeof(double) * (size_t)D);
+        return OR_STEP_YIELD;
+    }
+    if (s->time >= s->end) return OR_STEP_DONE;
+    if (s->time + s->dt >= s->end) { /* adams.rs:447-451 */
+        s->dt = s->end - s->time;
+        const int rc = adams_runge_kutta(s, 1);
+        if (rc) return rc;
+        *yt = s->time;
+        memcpy(yv, adams_v(s, s->vlen - 1), sizeof(double) * (size_t)D);
+        return OR_STEP_YIELD;
+    }
+    if (s->vlen == 0) { /* adams.rs:453-464 */
+        memcpy(s->save_state, s->state, sizeof(double) * (size_t)D);
+        if (s->time + s->dt * (s->order - 1.0) >= s->end)
+            s->dt = (s->end - s->time) / (s->order - 1.0);
+        const int rc = adams_runge_kutta(s, O - 1);
+        if (rc) return rc;
+        s->yield_memory = O;
+        return OR_STEP_REDO;
+    }
+    /* adams.rs:466-486 */
+    double scratch[MAXD], predictor[MAXD], corrector[MAXD], diff[MAXD];
+    for (int d = 0; d < D; ++d) scratch[d] = adams_d(s, 0)[d] * s->pred[O - 2];
+    for (int i = 1; i < O - 1; ++i) {
+        const double c = s->pred[O - i - 2];
+        for (int d = 0; d < D; ++d) scratch[d] = madd(pr, adams_d(s, i)[d], c, scratch[d]);
+    }
+    for (int d = 0; d < D; ++d) predictor[d] = madd(pr, scratch[d], s->dt, s->state[d]);
+    int rc = adams_rhs(s, s->time + s->dt, predictor, s->implicit);
+    if (rc) return rc;
+    for (int d = 0; d < D; ++d) scratch[d] = s->implicit[d] * s->corr[0];
+    for (int i = 0; i < O - 1; ++i) {
+        const double c = s->corr[O - i - 1];
+        for (int d = 0; d < D; ++d) scratch[d] = madd(pr, adams_d(s, i)[d], c, scratch[d]);
+    }
+    for (int d = 0; d < D; ++d) corrector[d] = madd(pr, scratch[d], s->dt, s->state[d]);
+    for (int d = 0; d < D; ++d) diff[d] = corrector[d] - predictor[d];
+    const double nrm = pr == OR_PROFILE_DEVICE ? sqrt(dev_sumsq(D, diff)) : nal_norm(D, diff);
+    const double error = s->err_coef / s->dt * nrm;
+    if (error <= s->tol) { /* adams.rs:490-525 */
+        memcpy(s->state, corrector, sizeof(double) * (size_t)D);
+        s->time += s->dt;
+        if (s->yield_memory == O) {
+            s->yield_memory -= 1;
+            return OR_STEP_REDO;
+        }
+        adams_push_d(s, s->implicit);
+        adams_push_v(s, s->time, s->state);
+        adams_pop_v(s);
+        adams_pop_d(s);
+        if (error < s->one_tenth * s->tol) {
+            const double q = adams_q(s, error);
+            if (q > s->four)
+                s->dt *= s->four;
+            else
+                s->dt *= q;
+            if (s->dt > s->dt_max) s->dt = s->dt_max;
+            s->vlen = s->vhead = 0;
+            s->dlen = s->dhead = 0;
+        }
+        *yt = s->time;
+        memcpy(yv, s->state, sizeof(double) * (size_t)D);
+        return OR_STEP_YIELD;
+    }
+    s->n_reject++;
+    if (s->yield_memory == O) { /* adams.rs:531-536 */
+        s->time -= s->dt * (s->order - 1.0);
+        memcpy(s->state, s->save_state, sizeof(double) * (size_t)D);
+    }
+    const double q = adams_q(s, error); /* adams.rs:538-556 */
+    if (q < s->one_tenth)
+        s->dt *= s->one_tenth;
+    else
+        s->dt *= q;
+    if (s->dt < s->dt_min) return OR_MINIMUM_TIME_DELTA_EXCEEDED;
+    s->vlen = s->vhead = 0;
+    s->dlen = s->dhead = 0;
+    return OR_STEP_REDO;
+}
+
+int32_t or_adams_coefficients(int32_t method, double *pred, double *corr, double *err) {
+    if (method != OR_ADAMS5 && method != OR_ADAMS3) return OR_INVALID_ARGUMENT;
+    or_problem p;
+    memset(&p, 0, sizeof p);
+    p.method = method;
+    p.dim = 1;
+    adams_solver *s = (adams_solver *)malloc(sizeof(adams_solver));
+    if (!s) return OR_INVALID_ARGUMENT;
+    double y0 = 0.0;
+    p.y0 = &y0;
+    adams_init(s, &p);
+    const int O = s->O;
+    memcpy(pred, s->pred, sizeof(double) * (size_t)O);
+    memcpy(corr, s->corr, sizeof(double) * (size_t)O);
+    *err = s->err_coef;
+    free(s);
+    return O;
+}
+
+double or_rootn(double x, int32_t n) { return dev_rootn(x, n); }
+
+static int adams_solve(const or_problem *p, double *y_out, or_result *r, or_path *path) {
+    adams_solver *s = (adams_solver *)malloc(sizeof(adams_solver));
+    if (!s) return OR_INVALID_ARGUMENT;
+    int rc = adams_init(s, p);
+    if (rc != OR_OK) {
+        free(s);
+        return rc;
+    }
+    const int64_t cap = p->max_attempts > 0 ? p->max_attempts : DEFAULT_MAX_ATTEMPTS;
+    int64_t calls = 0;
+    int status = OR_OK;
+    double yt = 0.0, yv[MAXD];
+    for (;;) {
+        if (calls >= cap) {
+            status = OR_MAXIMUM_ITERATIONS_EXCEEDED;
+            break;
+        }
+        calls++;
+        const int st = adams_step(s, &yt, yv);
+        if (st == OR_STEP_DONE) break;
+        if (st == OR_STEP_YIELD) {
+            r->n_yield++;
+            r->n_accepted++;
+            path_push(path, s->D, yt, yv, 0.0);
+        } else if (st != OR_STEP_REDO) {
+            status = st;
+            break;
+        }
+    }
+    r->t = s->time;
+    r->dt = s->dt;
+    r->status = status;
+    r->n_rhs = s->n_rhs;
+    r->n_rejected = s->n_reject;
+    memcpy(y_out, s->state, sizeof(double) * (size_t)s->D);
+    free(s);
+    return status;
+}
+
+/* ------------------------------------------------------------------ */
 /* public entry points                                                 */
 /* ------------------------------------------------------------------ */
 
@@ -1129,6 +1462,10 @@ int32_t or_solve(const or_problem *p, double *y_out, or_result *r, or_path *path
     if (p->method == OR_BDF6 || p->method == OR_BDF2) {
         if (p->n_stops) return OR_INVALID_ARGUMENT;
         return bdf_solve(p, y_out, r, path);
+    }
+    if (p->method == OR_ADAMS5 || p->method == OR_ADAMS3) {
+        if (p->n_stops) return OR_INVALID_ARGUMENT;
+        return adams_solve(p, y_out, r, path);
     }
     return OR_INVALID_ARGUMENT;
 }

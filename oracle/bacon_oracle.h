@@ -62,7 +62,7 @@ enum {
     OR_INVALID_ARGUMENT = -1
 };
 
-enum { OR_RK45 = 0, OR_RK23 = 1, OR_BDF6 = 2, OR_BDF2 = 3 };
+enum { OR_RK45 = 0, OR_RK23 = 1, OR_BDF6 = 2, OR_BDF2 = 3, OR_ADAMS5 = 4, OR_ADAMS3 = 5 };
 enum { OR_AS_BUILT = 0, OR_INTENDED = 1 };
 enum { OR_PROFILE_BACON = 0, OR_PROFILE_DEVICE = 1 };
 
@@ -166,6 +166,11 @@ int32_t or_lu_inverse(int32_t n, const double *a, double *inv);
 int32_t or_rk_tableau(int32_t method, int32_t semantics, double *c, double *m,
                       double *b, double *e);
 int32_t or_bdf_coefficients(int32_t method, double *higher, double *lower);
+/* Adams predictor / corrector coefficients and the error coefficient
+ * (adams.rs:568-661); returns the order O (5 or 3). */
+int32_t or_adams_coefficients(int32_t method, double *pred, double *corr, double *err);
+/* x^(1/n), n = 3 or 5, the DEVICE profile's exact-operation root */
+double or_rootn(double x, int32_t n);
 
 /* Discrete adjoint of the accepted RK45 (intended) step sequence and a frozen
  * step replay, used to check the GPU vjp.  See oracle/bacon_oracle.c. */

@@ -381,3 +381,103 @@ def test_builder_validation_order(orc):
     assert b.call("with_ending_time", 1.0) == O.OK
     assert b.call("with_initial_time", 1.0) == O.TIME_START_OOB
     assert b.call("solve") == O.MISSING_PARAMETERS
+
+
+# ---------------------------------------------------------------- Adams
+ADAMS_CASES = [
+    # method, rhs, y0, t_end, dt_min, dt_max, tol, exact    (adams.rs:696-922)
+    ("adams5_exp", O.ADAMS5, "exp", 1.0, 2.0, 1e-5, 0.1, 0.0005, np.exp),
+    ("adams5_quadratic", O.ADAMS5, "quadratic", 1.0, 5.0, 1e-7, 0.001, 0.01,
+     lambda t: 1.0 - t * t),
+    ("adams5_sine", O.ADAMS5, "cos", 0.0, 2 * np.pi, 1e-5, 0.001, 0.01, np.sin),
+    ("adams3_exp", O.ADAMS3, "exp", 1.0, 2.0, 1e-5, 0.1, 0.001, np.exp),
+    ("adams3_quadratic", O.ADAMS3, "quadratic", 1.0, 5.0, 1e-5, 0.001, 0.1,
+     lambda t: 1.0 - t * t),
+    ("adams3_sine", O.ADAMS3, "cos", 0.0, 2 * np.pi, 1e-5, 0.001, 0.01, np.sin),
+]
+
+
+@pytest.mark.parametrize("profile", [O.PROFILE_BACON, O.PROFILE_DEVICE])
+@pytest.mark.parametrize("semantics", [O.AS_BUILT, O.INTENDED])
+@pytest.mark.parametrize("case", ADAMS_CASES, ids=[c[0] for c in ADAMS_CASES])
+def test_reference_adams(orc, case, semantics, profile):
+    """The reference's Adams tests: every yielded point within 0.01 of the
+    closed form (float_cmp approx_eq), and the solve reaches t_end."""
+    name, method, rhs, y0, t_end, dt_min, dt_max, tol, exact = case
+    r = O.solve(method, rhs, [y0], 0.0, t_end, tol, dt_min, dt_max,
+                semantics=semantics, profile=profile, path_cap=200000)
+    assert r["status"] == O.OK
+    assert r["path_len"] > 0
+    assert abs(r["t"] - t_end) < 1e-9
+    for t, y in zip(r["path_t"], r["path_y"]):
+        assert approx_eq(y[0], exact(t), 0.01)
+
+
+@pytest.mark.parametrize("method,order", [(O.ADAMS5, 5), (O.ADAMS3, 3)])
+def test_adams_coefficients_order_conditions(orc, method, order):
+    """Predictor: Adams-Bashforth with (O-1) steps, weight pred[j] on
+    f_{n-j}: sum_j pred[j] (-j)^q = 1/(q+1) for q < O-1.  Corrector:
+    Adams-Moulton, corr[0] on f_{n+1}, corr[k] on f_{n+1-k}:
+    sum_k corr[k] (1-k)^q = 1/(q+1) for q < O.  Error coefficient 19/270."""
+    pred, corr, err = O.adams_coefficients(method)
+    assert pred.size == order and corr.size == order
+    for q in range(order - 1):
+        s = sum(pred[j] * (-j) ** q for j in range(order - 1))
+        assert abs(s - 1.0 / (q + 1)) < 1e-14, (q, s)
+    assert pred[order - 1] == 0.0
+    for q in range(order):
+        s = sum(corr[k] * (1 - k) ** q for k in range(order))
+        assert abs(s - 1.0 / (q + 1)) < 1e-14, (q, s)
+    assert err == 19.0 / 270.0
+
+
+@pytest.mark.parametrize("n", [3, 5])
+def test_rootn_accuracy(orc, n):
+    rng = np.random.default_rng(n)
+    xs = np.concatenate([np.exp(rng.uniform(-690, 690, 4000)), [1.0, 2.0, 0.5, 7.0, 1e-300]])
+    for x in xs:
+        got = O.rootn(x, n)
+        want = x ** (1.0 / n)
+        assert abs(got / want - 1.0) < 3e-7, (x, got, want)
+    assert O.rootn(0.0, n) == 0.0 and O.rootn(np.inf, n) == np.inf
+    assert np.isnan(O.rootn(np.nan, n))
+
+
+def test_adams_as_built_derivative_history_is_one_step_stale(orc):
+    """As built, the first Adams step after an RK4 bootstrap is held back
+    while the bootstrap values are yielded, and its derivative never joins
+    the derivative deque (adams.rs:434-441 pushes the value only), so the
+    next predictor and corrector use a history one step behind.  On the
+    reference's scalar tests that still passes (above); on Lorenz-63 at tol
+    1e-6 the error estimate then fails every step, dt collapses below dt_min
+    and the solve ends with MinimumTimeDeltaExceeded near t = 0.32 after
+    millions of RK4 bootstrap yields.  The intended variant (the derivative
+    joins the history) finishes with 98 rejections."""
+    kw = dict(t0=0.0, t_end=10.0, tol=1e-6, dt_min=1e-8, dt_max=0.1)
+    a = O.solve(O.ADAMS5, "lorenz", [1.0, 1.0, 20.0], params=[10.0, 28.0, 8.0 / 3.0],
+                semantics=O.AS_BUILT, **kw)
+    b = O.solve(O.ADAMS5, "lorenz", [1.0, 1.0, 20.0], params=[10.0, 28.0, 8.0 / 3.0],
+                semantics=O.INTENDED, **kw)
+    assert a["status"] == O.MINIMUM_TIME_DELTA_EXCEEDED
+    assert a["t"] < 0.5 and a["n_accepted"] > 10 ** 6
+    assert b["status"] == O.OK and b["t"] == 10.0
+    assert b["n_rejected"] < 200
+
+
+def test_adams_lorenz_against_golden(orc):
+    """Adams5 and Adams3 (intended) on golden Lorenz instances against the
+    DOP853 1e-13 solution at t = 2: errors shrink with the tolerance, and
+    Adams5 at tol 1e-10 is within 1e-9."""
+    g = np.load("tests/golden/lorenz_golden.npz")
+    # Adams3 (2nd order) is held to tol 1e-8: at 1e-10 it needs ~10^7 steps
+    for method, tols in ((O.ADAMS5, (1e-6, 1e-8, 1e-10)), (O.ADAMS3, (1e-6, 1e-8))):
+        for i in range(4):
+            errs = []
+            for tol in tols:
+                r = O.solve(method, "lorenz", g["y0"][:, i], 0.0, g["times"][2], tol, 1e-12,
+                            0.1, params=g["params"][:, i], semantics=O.INTENDED)
+                assert r["status"] == O.OK
+                errs.append(np.abs(r["y"] - g["y"][i, 2]).max())
+            assert all(a > b for a, b in zip(errs, errs[1:])), (method, i, errs)
+            if method == O.ADAMS5:
+                assert errs[-1] < 1e-9, (i, errs)
