@@ -18,7 +18,7 @@ ERR_HIP = -2
 ERR_UNSUPPORTED = -3
 ERR_STATE = -4
 
-RK45, RK23, BDF6, BDF2 = 0, 1, 2, 3
+RK45, RK23, BDF6, BDF2, ADAMS5, ADAMS3 = 0, 1, 2, 3, 4, 5
 AS_BUILT, INTENDED = 0, 1
 DIM_STATIC, DIM_DYN = 0, 1
 
@@ -152,6 +152,8 @@ SIGNATURES = [
      [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.c_void_p]),
     ("bg_bdf_solve_device", C.c_int,
      [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.c_void_p]),
+    ("bg_adams_solve_device", C.c_int,
+     [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.c_void_p]),
     ("bg_rk_vjp_device", C.c_int,
      [C.c_int, C.c_int, C.POINTER(AdjointDesc), C.c_void_p]),
     ("bg_version", C.c_int, []),

@@ -34,7 +34,8 @@ int device_cu_count(int dev) {
 
 static bool is_rk(int m) { return m == BG_RK45 || m == BG_RK23; }
 static bool is_bdf(int m) { return m == BG_BDF6 || m == BG_BDF2; }
-static bool valid_method(int m) { return is_rk(m) || is_bdf(m); }
+static bool is_adams(int m) { return m == BG_ADAMS5 || m == BG_ADAMS3; }
+static bool valid_method(int m) { return is_rk(m) || is_bdf(m) || is_adams(m); }
 
 // number of per-instance parameters a registered RHS reads
 static int rhs_param_count(int rhs) {
@@ -297,6 +298,8 @@ extern "C" int bg_solve(bg_solver *s) {
     s->solved = false;
     s->fused = false;
     s->next = 0;
+    // Adams runs batched only (bg_ensemble_*, bg_adams_solve_device)
+    if (is_adams(s->method)) return BG_ERR_UNSUPPORTED;
     int rc = is_rk(s->method) ? solve_rk_path(s) : solve_bdf_path(s);
     if (rc != BG_OK) return rc;
     s->solved = true;
@@ -585,6 +588,8 @@ extern "C" int bg_ensemble_solve(bg_ensemble *e) {
         bg_launch lc;
         memset(&lc, 0, sizeof lc);
         rc = rk_launch_ensemble(e->method, e->rhs, e->semantics, d, lc, nullptr);
+    } else if (is_adams(e->method)) {
+        rc = adams_launch_ensemble(e->method, e->rhs, e->semantics, d, nullptr);
     } else {
         rc = bdf_launch_ensemble(e->method, e->rhs, e->semantics, e->lu_bits == 32, d, nullptr);
     }
@@ -684,6 +689,12 @@ extern "C" int bg_bdf_solve_device(int method, int rhs, int semantics, int lu_bi
                                    const bg_ensemble_desc *d, void *stream) {
     if (!d || !is_bdf(method) || (lu_bits != 32 && lu_bits != 64)) return BG_ERR_INVALID_ARGUMENT;
     return bdf_launch_ensemble(method, rhs, semantics, lu_bits == 32, *d, (hipStream_t)stream);
+}
+
+extern "C" int bg_adams_solve_device(int method, int rhs, int semantics,
+                                     const bg_ensemble_desc *d, void *stream) {
+    if (!d || !is_adams(method)) return BG_ERR_INVALID_ARGUMENT;
+    return adams_launch_ensemble(method, rhs, semantics, *d, (hipStream_t)stream);
 }
 
 extern "C" int bg_rk_vjp_device(int method, int rhs, const bg_adjoint_desc *d, void *stream) {

@@ -24,6 +24,11 @@ bool rk_supported(int method, int rhs, uint32_t dim);
 int rk_launch_path_ensemble(int method, int rhs, int semantics, const bg_ensemble_desc &d,
                             const bg_path_desc &p, hipStream_t stream);
 
+// Adams family (adams_kernels.hip)
+bool adams_supported(int method, int rhs, uint32_t dim);
+int adams_launch_ensemble(int method, int rhs, int semantics, const bg_ensemble_desc &d,
+                          hipStream_t stream, void *ws = nullptr);
+
 // Autotuner (tune.hip)
 int rk_autotune(int method, int rhs, int semantics, const bg_ensemble_desc &d, bg_tuning &tn,
                 hipStream_t stream);

@@ -54,6 +54,8 @@ extern "C" {
 #define BG_RK23 1 /* RungeKutta23, rk.rs:656 */
 #define BG_BDF6 2 /* BDF6, bdf.rs:706        */
 #define BG_BDF2 3 /* BDF2, bdf.rs:762        */
+#define BG_ADAMS5 4 /* Adams5, adams.rs:633  (ensembles)  */
+#define BG_ADAMS3 5 /* Adams3, adams.rs:693  (ensembles)  */
 
 /* ---- semantics: the reference's quirk register as data ------------------
  * BG_AS_BUILT : the shipped behaviour (RK stage matrix read transposed,
@@ -284,6 +286,13 @@ void bg_graph_destroy(bg_graph *g);
  * factored in FP32, every residual and the Broyden iteration in FP64. */
 int bg_bdf_solve_device(int method, int rhs, int semantics, int lu_bits,
                         const bg_ensemble_desc *d, void *stream);
+
+/* Batched Adams-Bashforth-Moulton (Adams5 / Adams3) in one persistent
+ * launch on `stream` (asynchronous).  Semantics: BG_AS_BUILT keeps the
+ * reference's derivative history, which misses the first Adams step after
+ * each RK4 bootstrap (adams.rs:434-441); BG_INTENDED adds it. */
+int bg_adams_solve_device(int method, int rhs, int semantics, const bg_ensemble_desc *d,
+                          void *stream);
 
 /* Discrete adjoint of the accepted RK step sequence (RK45, intended tableau)
  * for the loss L = sum_k |y(t_k) - target_k|^2 over observation times t_k
