@@ -126,6 +126,7 @@ SIGNATURES = [
     ("bg_ensemble_with_max_attempts", C.c_int, [C.c_void_p, C.c_uint64]),
     ("bg_ensemble_with_lu_precision", C.c_int, [C.c_void_p, C.c_int]),
     ("bg_ensemble_solve", C.c_int, [C.c_void_p]),
+    ("bg_ensemble_solve_ivp", C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
     ("bg_ensemble_solve_plain", C.c_int, [C.c_void_p]),
     ("bg_ensemble_results", C.c_int,
      [C.c_void_p, _dp, _dp, C.POINTER(C.c_int32), C.POINTER(Counts)]),

@@ -612,6 +612,115 @@ extern "C" int bg_ensemble_solve_plain(bg_ensemble *e) {
     return BG_OK;
 }
 
+// README.md:45-47: solve_ivp tries a fifth-order predictor-corrector, then
+// Runge-Kutta-Fehlberg, then BDF6.  Batched: every instance starts with
+// Adams5; the instances that did not finish OK are gathered into a new
+// ensemble for RK45, and those still failing into one for BDF6.  A method
+// that does not support the right-hand side is skipped.
+static bool method_supports(int method, int rhs, uint32_t dim) {
+    if (is_adams(method)) return adams_supported(method, rhs, dim);
+    if (is_rk(method)) return rhs == BG_RHS_LINEAR ? dim == 128 && method == BG_RK45
+                                                   : rk_supported(method, rhs, dim);
+    return bdf_supported(method, rhs, dim);
+}
+
+static bg_ensemble *ensemble_subset(const bg_ensemble *e, const std::vector<uint64_t> &idx,
+                                    int method) {
+    bg_ensemble *s = new bg_ensemble();
+    const uint64_t m = idx.size(), n = e->n;
+    s->method = method;
+    s->dim = e->dim;
+    s->n = m;
+    s->semantics = e->semantics;
+    s->rhs = e->rhs;
+    s->n_params = e->n_params;
+    s->max_attempts = e->max_attempts;
+    s->lu_bits = e->lu_bits;
+    auto sub = [&](const Field &f, Field &g) {
+        g.set = f.set;
+        g.scalar = f.scalar;
+        if (!f.v.empty()) {
+            g.v.resize(m);
+            for (uint64_t j = 0; j < m; ++j) g.v[j] = f.v[idx[j]];
+        }
+    };
+    sub(e->tol, s->tol);
+    sub(e->dt_max, s->dt_max);
+    sub(e->dt_min, s->dt_min);
+    sub(e->t0, s->t0);
+    sub(e->t_end, s->t_end);
+    s->y0.resize((size_t)e->dim * m);
+    for (uint32_t c = 0; c < e->dim; ++c)
+        for (uint64_t j = 0; j < m; ++j) s->y0[c * m + j] = e->y0[c * n + idx[j]];
+    s->params.resize((size_t)e->n_params * m);
+    for (uint32_t k = 0; k < e->n_params; ++k)
+        for (uint64_t j = 0; j < m; ++j) s->params[k * m + j] = e->params[k * n + idx[j]];
+    s->shared = e->shared;
+    s->have_y0 = e->have_y0;
+    s->have_deriv = e->have_deriv;
+    return s;
+}
+
+extern "C" int bg_ensemble_solve_ivp(bg_ensemble *e, int32_t *method_used) {
+    if (!e) return BG_ERR_INVALID_ARGUMENT;
+    if (!e->have_y0 || !e->have_deriv || !e->tol.set || !e->dt_max.set || !e->dt_min.set ||
+        !e->t0.set || !e->t_end.set)
+        return BG_MISSING_PARAMETERS;
+    const uint64_t n = e->n;
+    const uint32_t D = e->dim;
+    std::vector<double> t(n, 0.0), y((size_t)D * n);
+    std::vector<int32_t> st(n, BG_ERR_UNSUPPORTED), used(n, -1);
+    std::vector<bg_counts> cnt(n);
+    memset(cnt.data(), 0, n * sizeof(bg_counts));
+    for (uint64_t i = 0; i < n; ++i)
+        for (uint32_t c = 0; c < D; ++c) y[c * n + i] = e->y0[c * n + i];
+    std::vector<uint64_t> todo(n);
+    for (uint64_t i = 0; i < n; ++i) todo[i] = i;
+    const int chain[3] = {BG_ADAMS5, BG_RK45, BG_BDF6};
+    for (int method : chain) {
+        if (todo.empty()) break;
+        if (!method_supports(method, e->rhs, D)) continue;
+        bg_ensemble *s = ensemble_subset(e, todo, method);
+        const uint64_t m = todo.size();
+        std::vector<double> ts(m), ys((size_t)D * m);
+        std::vector<int32_t> ss(m);
+        std::vector<bg_counts> cs(m);
+        int rc = bg_ensemble_solve(s);
+        if (rc == BG_OK) rc = bg_ensemble_results(s, ts.data(), ys.data(), ss.data(), cs.data());
+        delete s;
+        if (rc != BG_OK) return rc;
+        std::vector<uint64_t> next;
+        for (uint64_t j = 0; j < m; ++j) {
+            const uint64_t i = todo[j];
+            t[i] = ts[j];
+            for (uint32_t c = 0; c < D; ++c) y[c * n + i] = ys[c * m + j];
+            st[i] = ss[j];
+            cnt[i] = cs[j];
+            used[i] = method;
+            if (ss[j] != BG_OK) next.push_back(i);
+        }
+        todo.swap(next);
+    }
+    // results into the ensemble's own buffers
+    int rc;
+    if ((rc = e->d_tout.ensure(n * sizeof(double))) != BG_OK) return rc;
+    if ((rc = e->d_yout.ensure((size_t)D * n * sizeof(double))) != BG_OK) return rc;
+    if ((rc = e->d_status.ensure(n * sizeof(int32_t))) != BG_OK) return rc;
+    if ((rc = e->d_counts.ensure(n * sizeof(bg_counts))) != BG_OK) return rc;
+    if (n) {
+        BG_HIP_CHECK(hipMemcpy(e->d_tout.p, t.data(), n * sizeof(double), hipMemcpyHostToDevice));
+        BG_HIP_CHECK(hipMemcpy(e->d_yout.p, y.data(), (size_t)D * n * sizeof(double),
+                               hipMemcpyHostToDevice));
+        BG_HIP_CHECK(hipMemcpy(e->d_status.p, st.data(), n * sizeof(int32_t),
+                               hipMemcpyHostToDevice));
+        BG_HIP_CHECK(hipMemcpy(e->d_counts.p, cnt.data(), n * sizeof(bg_counts),
+                               hipMemcpyHostToDevice));
+    }
+    if (method_used) memcpy(method_used, used.data(), n * sizeof(int32_t));
+    e->solved = true;
+    return BG_OK;
+}
+
 extern "C" int bg_ensemble_results(bg_ensemble *e, double *t_out, double *y_out,
                                    int32_t *status, bg_counts *counts) {
     if (!e) return BG_ERR_INVALID_ARGUMENT;

@@ -280,6 +280,16 @@ class Ensemble:
         _check(fn(self._h))
         return self.results()
 
+    def solve_ivp(self):
+        """README.md:45-47 chain per instance: Adams5 -> RK45 -> BDF6.
+        Returns the results plus `method_used` per instance."""
+        used = np.full(self.n, -1, dtype=np.int32)
+        _check(L.lib().bg_ensemble_solve_ivp(self._h,
+                                             used.ctypes.data_as(C.POINTER(C.c_int32))))
+        out = self.results()
+        out["method_used"] = used
+        return out
+
     def results(self):
         t = np.zeros(self.n)
         y = np.zeros((self.dim_, self.n))

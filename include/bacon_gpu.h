@@ -160,6 +160,13 @@ int bg_ensemble_solve(bg_ensemble *e);
 /* Plain path: the same device step code launched once per instance,
  * host-synchronous, one instance after another. */
 int bg_ensemble_solve_plain(bg_ensemble *e);
+/* solve_ivp (README.md:45-47) per instance: Adams5, then RK45 for the
+ * instances that did not finish OK, then BDF6 for those still failing
+ * (methods that do not support the right-hand side are skipped).  The
+ * ensemble's own method is not used; its semantics are.  method_used[n]
+ * (may be NULL) receives the method of each instance's final result, -1 if
+ * none applied.  Results as from bg_ensemble_solve. */
+int bg_ensemble_solve_ivp(bg_ensemble *e, int32_t *method_used);
 /* Copy results to host arrays (any may be NULL); y_out is [dim][n]. */
 int bg_ensemble_results(bg_ensemble *e, double *t_out, double *y_out, int32_t *status,
                         bg_counts *counts);
