@@ -102,15 +102,21 @@ struct AdamsCoef<3> {
 
 enum : int { ADAMS_YIELD = -1, ADAMS_REDO = -2, ADAMS_DONE = -3 };
 
-template <class RHS, int O, bool AsBuilt>
+// PATH: also keep the values of the last bootstrap (the value deque's
+// entries, oldest first), which the yield_memory countdown of adams.rs:415-428
+// yields; `yitem` names what a yield returns: -1 the current (t, y), else
+// that entry.  Without PATH both are dead and compile away.
+template <class RHS, int O, bool AsBuilt, bool PATH = false>
 struct AdamsLane {
     static constexpr int D = RHS::D;
     using C = AdamsCoef<O>;
     double t, dt, end, tol, dt_min, dt_max;
     double y[D], save[D], impl[D];
     double dq[O - 1][D];  // derivative history, oldest first (shift register)
+    double vt[PATH ? O - 1 : 1], vy[PATH ? O - 1 : 1][D];  // value deque (PATH only)
     int vlen;             // length of the value deque
     int ym;               // yield_memory
+    int yitem;
     double p[RHS::P > 0 ? RHS::P : 1];
     uint32_t n_rej;
 
@@ -124,6 +130,20 @@ struct AdamsLane {
             for (int c = 0; c < D; ++c) dq[k][c] = dq[k + 1][c];
 #pragma unroll
         for (int c = 0; c < D; ++c) dq[O - 2][c] = v[c];
+    }
+    // value push at the full deque: push_back + pop_front, as for dq
+    __device__ __forceinline__ void push_v() {
+        if constexpr (PATH) {
+#pragma unroll
+            for (int k = 0; k < O - 2; ++k) {
+                vt[k] = vt[k + 1];
+#pragma unroll
+                for (int c = 0; c < D; ++c) vy[k][c] = vy[k + 1][c];
+            }
+            vt[O - 2] = t;
+#pragma unroll
+            for (int c = 0; c < D; ++c) vy[O - 2][c] = y[c];
+        }
     }
 
     // adams.rs:344-396
@@ -154,6 +174,7 @@ struct AdamsLane {
                 double fd[D];
                 if (!f(t, y, fd)) return BG_USER_ERROR;
                 push_d(fd);
+                push_v();
                 ++vlen;
             }
 #pragma unroll
@@ -168,6 +189,7 @@ struct AdamsLane {
         double fd[D];
         if (!f(t, y, fd)) return BG_USER_ERROR;
         push_d(fd);
+        push_v();
         ++vlen;
         return BG_OK;
     }
@@ -178,7 +200,9 @@ struct AdamsLane {
 
     // One AdamsSolver::step (adams.rs:411-561)
     __device__ __forceinline__ int step() {
+        yitem = -1;
         if (ym > 0 && ym < O) {  // adams.rs:415-428
+            yitem = O - ym - 1;
             ym -= 1;
             if (ym == 0) ym = O + 1;
             return ADAMS_YIELD;
@@ -355,6 +379,108 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// Single-instance path production: every yield of the iterator (adams.rs:
+// 398-565 through ivp.rs:205-239) into the path buffer; stops when the
+// buffer is full (resumable: the lane state goes back to *st) or the solve
+// ends.  Steps are counted like the ensemble kernel's, as step() calls.
+template <class RHS, int O, bool AsBuilt>
+__global__ void __launch_bounds__(64)
+    adams_path_kernel(AdamsPathState *__restrict__ st, double *__restrict__ path_t,
+                      double *__restrict__ path_y, uint32_t cap) {
+    constexpr int D = RHS::D;
+    static_assert(D <= 4 && O - 1 <= 4 && RHS::P <= 8, "AdamsPathState sizes");
+    if (threadIdx.x != 0) return;
+    AdamsLane<RHS, O, AsBuilt, true> L;
+    L.t = st->t;
+    L.dt = st->dt;
+    L.end = st->end;
+    L.tol = st->tol;
+    L.dt_min = st->dt_min;
+    L.dt_max = st->dt_max;
+    L.vlen = st->vlen;
+    L.ym = st->ym;
+    L.yitem = -1;
+    L.n_rej = st->n_rejected;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        L.y[c] = st->y[c];
+        L.save[c] = st->save[c];
+        L.impl[c] = st->impl[c];
+    }
+#pragma unroll
+    for (int k = 0; k < O - 1; ++k) {
+        L.vt[k] = st->vt[k];
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            L.dq[k][c] = st->dq[k][c];
+            L.vy[k][c] = st->vy[k][c];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RHS::P; ++k) L.p[k] = st->p[k];
+    uint64_t steps = st->steps;
+    const uint64_t max_steps = st->max_steps;
+    uint32_t nacc = st->n_accepted, count = 0;
+    int status = st->status, finished = st->finished;
+    while (!finished && count < cap) {
+        if (steps >= max_steps) {
+            status = BG_MAXIMUM_ITERATIONS_EXCEEDED;
+            finished = 1;
+            break;
+        }
+        ++steps;
+        const int r = L.step();
+        if (r == ADAMS_YIELD) {
+            ++nacc;
+            double yt = L.t, yv[D];
+#pragma unroll
+            for (int c = 0; c < D; ++c) yv[c] = L.y[c];
+#pragma unroll
+            for (int k = 0; k < O - 1; ++k) {
+                if (L.yitem == k) {
+                    yt = L.vt[k];
+#pragma unroll
+                    for (int c = 0; c < D; ++c) yv[c] = L.vy[k][c];
+                }
+            }
+            path_t[count] = yt;
+#pragma unroll
+            for (int c = 0; c < D; ++c) path_y[(uint64_t)count * D + c] = yv[c];
+            ++count;
+        } else if (r == ADAMS_DONE) {
+            finished = 1;
+        } else if (r != ADAMS_REDO) {
+            status = r;
+            finished = 1;
+        }
+    }
+    st->t = L.t;
+    st->dt = L.dt;
+    st->vlen = L.vlen;
+    st->ym = L.ym;
+    st->n_rejected = L.n_rej;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+        st->y[c] = L.y[c];
+        st->save[c] = L.save[c];
+        st->impl[c] = L.impl[c];
+    }
+#pragma unroll
+    for (int k = 0; k < O - 1; ++k) {
+        st->vt[k] = L.vt[k];
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            st->dq[k][c] = L.dq[k][c];
+            st->vy[k][c] = L.vy[k][c];
+        }
+    }
+    st->steps = steps;
+    st->n_accepted = nacc;
+    st->status = status;
+    st->finished = finished;
+    st->count = count;
+}
+
 __global__ void adams_setup_kernel(uint32_t *counter, bg_ensemble_desc *dst,
                                    const bg_ensemble_desc d) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -427,6 +553,18 @@ int adams_launch_ensemble(int method, int rhs, int semantics, const bg_ensemble_
         adams_ensemble_kernel<R, O, AB><<<(uint32_t)grid, 256, 0, stream>>>(ddesc, counter);
         BG_HIP_CHECK(hipGetLastError());
         if (!ws_in) BG_HIP_CHECK(hipFreeAsync(ws, stream));
+        return BG_OK;
+    });
+}
+
+int adams_launch_path(int method, int rhs, int semantics, uint32_t dim, AdamsPathState *state,
+                      double *path_t, double *path_y, uint32_t cap, hipStream_t stream) {
+    return visit_adams(method, rhs, semantics, dim, [&](auto rtag, auto otag, auto atag) -> int {
+        using R = typename decltype(rtag)::type;
+        constexpr int O = decltype(otag)::value;
+        constexpr bool AB = decltype(atag)::value;
+        adams_path_kernel<R, O, AB><<<1, 64, 0, stream>>>(state, path_t, path_y, cap);
+        BG_HIP_CHECK(hipGetLastError());
         return BG_OK;
     });
 }

@@ -181,10 +181,53 @@ extern "C" int bg_with_dt_max(bg_solver *s, double v) { return bg_with_maximum_d
 extern "C" int bg_with_start(bg_solver *s, double v) { return bg_with_initial_time(s, v); }
 extern "C" int bg_with_end(bg_solver *s, double v) { return bg_with_ending_time(s, v); }
 
-static int solve_rk_path(bg_solver *s) {
-    if (!rk_supported(s->method, s->rhs, s->dim)) return BG_ERR_UNSUPPORTED;
+// Drive a resumable single-instance path kernel: launch, drain the path
+// buffer into the solver's host path, relaunch until the state says finished.
+template <class State, class Launch>
+static int drain_path(bg_solver *s, const State &init, Launch &&launch) {
     const uint32_t cap = 1u << 16;
     const uint32_t D = s->dim;
+    State host = init;
+    State *dstate = nullptr;
+    double *dpt = nullptr, *dpy = nullptr;
+    int rc = BG_OK;
+    if (hipMalloc((void **)&dstate, sizeof(State)) != hipSuccess ||
+        hipMalloc((void **)&dpt, sizeof(double) * cap) != hipSuccess ||
+        hipMalloc((void **)&dpy, sizeof(double) * cap * D) != hipSuccess)
+        rc = BG_ERR_HIP;
+    if (rc == BG_OK && hipMemcpy(dstate, &host, sizeof host, hipMemcpyHostToDevice) != hipSuccess)
+        rc = BG_ERR_HIP;
+    s->path_t.clear();
+    s->path_y.clear();
+    while (rc == BG_OK) {
+        rc = launch(dstate, dpt, dpy, cap);
+        if (rc != BG_OK) break;
+        if (hipMemcpy(&host, dstate, sizeof host, hipMemcpyDeviceToHost) != hipSuccess) {
+            rc = BG_ERR_HIP;
+            break;
+        }
+        const size_t old = s->path_t.size();
+        s->path_t.resize(old + host.count);
+        s->path_y.resize((old + host.count) * D);
+        if (host.count &&
+            (hipMemcpy(s->path_t.data() + old, dpt, sizeof(double) * host.count,
+                       hipMemcpyDeviceToHost) != hipSuccess ||
+             hipMemcpy(s->path_y.data() + old * D, dpy, sizeof(double) * host.count * D,
+                       hipMemcpyDeviceToHost) != hipSuccess)) {
+            rc = BG_ERR_HIP;
+            break;
+        }
+        if (host.finished) break;
+    }
+    if (dstate) hipFree(dstate);
+    if (dpt) hipFree(dpt);
+    if (dpy) hipFree(dpy);
+    if (rc == BG_OK) s->failure = host.status;
+    return rc;
+}
+
+static int solve_rk_path(bg_solver *s) {
+    if (!rk_supported(s->method, s->rhs, s->dim)) return BG_ERR_UNSUPPORTED;
     PathState host;
     memset(&host, 0, sizeof host);
     host.t = s->t0;
@@ -194,52 +237,15 @@ static int solve_rk_path(bg_solver *s) {
     host.dt_max = s->dt_max;
     host.dt = (s->dt_max + s->dt_min) * 0.5;  // rk.rs:315
     host.max_attempts = 10000000ull;
-    for (uint32_t k = 0; k < D; ++k) host.y[k] = s->y0[k];
+    for (uint32_t k = 0; k < s->dim; ++k) host.y[k] = s->y0[k];
     for (size_t k = 0; k < s->params.size() && k < 8; ++k) host.p[k] = s->params[k];
-    PathState *dstate = nullptr;
-    double *dpt = nullptr, *dpy = nullptr;
-    int rc = BG_OK;
-    hipStream_t stream = nullptr;
-    if (hipMalloc((void **)&dstate, sizeof(PathState)) != hipSuccess ||
-        hipMalloc((void **)&dpt, sizeof(double) * cap) != hipSuccess ||
-        hipMalloc((void **)&dpy, sizeof(double) * cap * D) != hipSuccess) {
-        rc = BG_ERR_HIP;
-    }
-    if (rc == BG_OK && hipMemcpy(dstate, &host, sizeof host, hipMemcpyHostToDevice) != hipSuccess)
-        rc = BG_ERR_HIP;
-    s->path_t.clear();
-    s->path_y.clear();
-    while (rc == BG_OK) {
-        rc = rk_launch_path(s->method, s->rhs, s->semantics, D, dstate, dpt, dpy, cap, stream);
-        if (rc != BG_OK) break;
-        if (hipMemcpy(&host, dstate, sizeof host, hipMemcpyDeviceToHost) != hipSuccess) {
-            rc = BG_ERR_HIP;
-            break;
-        }
-        const size_t old = s->path_t.size();
-        s->path_t.resize(old + host.count);
-        s->path_y.resize((old + host.count) * D);
-        if (host.count &&
-            (hipMemcpy(s->path_t.data() + old, dpt, sizeof(double) * host.count,
-                       hipMemcpyDeviceToHost) != hipSuccess ||
-             hipMemcpy(s->path_y.data() + old * D, dpy, sizeof(double) * host.count * D,
-                       hipMemcpyDeviceToHost) != hipSuccess)) {
-            rc = BG_ERR_HIP;
-            break;
-        }
-        if (host.finished) break;
-    }
-    if (dstate) hipFree(dstate);
-    if (dpt) hipFree(dpt);
-    if (dpy) hipFree(dpy);
-    if (rc == BG_OK) s->failure = host.status;
-    return rc;
+    return drain_path(s, host, [&](PathState *st, double *pt, double *py, uint32_t cap) {
+        return rk_launch_path(s->method, s->rhs, s->semantics, s->dim, st, pt, py, cap, nullptr);
+    });
 }
 
 static int solve_bdf_path(bg_solver *s) {
     if (!bdf_supported(s->method, s->rhs, s->dim)) return BG_ERR_UNSUPPORTED;
-    const uint32_t cap = 1u << 16;
-    const uint32_t D = s->dim;
     BdfPathState host;
     memset(&host, 0, sizeof host);
     host.t = s->t0;
@@ -249,58 +255,44 @@ static int solve_bdf_path(bg_solver *s) {
     host.dt_max = s->dt_max;
     host.dt = (s->dt_max + s->dt_min) * 0.5;  // bdf.rs:302
     host.max_steps = 1000000ull;
-    for (uint32_t k = 0; k < D; ++k) host.y[k] = host.save[k] = s->y0[k];
+    for (uint32_t k = 0; k < s->dim; ++k) host.y[k] = host.save[k] = s->y0[k];
     for (size_t k = 0; k < s->params.size() && k < 8; ++k) host.p[k] = s->params[k];
-    BdfPathState *dstate = nullptr;
-    double *dpt = nullptr, *dpy = nullptr;
-    int rc = BG_OK;
-    if (hipMalloc((void **)&dstate, sizeof(BdfPathState)) != hipSuccess ||
-        hipMalloc((void **)&dpt, sizeof(double) * cap) != hipSuccess ||
-        hipMalloc((void **)&dpy, sizeof(double) * cap * D) != hipSuccess)
-        rc = BG_ERR_HIP;
-    if (rc == BG_OK && hipMemcpy(dstate, &host, sizeof host, hipMemcpyHostToDevice) != hipSuccess)
-        rc = BG_ERR_HIP;
-    s->path_t.clear();
-    s->path_y.clear();
-    while (rc == BG_OK) {
-        rc = bdf_launch_path(s->method, s->rhs, s->semantics, D, dstate, dpt, dpy, cap, nullptr);
-        if (rc != BG_OK) break;
-        if (hipMemcpy(&host, dstate, sizeof host, hipMemcpyDeviceToHost) != hipSuccess) {
-            rc = BG_ERR_HIP;
-            break;
-        }
-        const size_t old = s->path_t.size();
-        s->path_t.resize(old + host.count);
-        s->path_y.resize((old + host.count) * D);
-        if (host.count &&
-            (hipMemcpy(s->path_t.data() + old, dpt, sizeof(double) * host.count,
-                       hipMemcpyDeviceToHost) != hipSuccess ||
-             hipMemcpy(s->path_y.data() + old * D, dpy, sizeof(double) * host.count * D,
-                       hipMemcpyDeviceToHost) != hipSuccess)) {
-            rc = BG_ERR_HIP;
-            break;
-        }
-        if (host.finished) break;
-    }
-    if (dstate) hipFree(dstate);
-    if (dpt) hipFree(dpt);
-    if (dpy) hipFree(dpy);
-    if (rc == BG_OK) s->failure = host.status;
-    return rc;
+    return drain_path(s, host, [&](BdfPathState *st, double *pt, double *py, uint32_t cap) {
+        return bdf_launch_path(s->method, s->rhs, s->semantics, s->dim, st, pt, py, cap, nullptr);
+    });
+}
+
+static int solve_adams_path(bg_solver *s) {
+    if (!adams_supported(s->method, s->rhs, s->dim)) return BG_ERR_UNSUPPORTED;
+    AdamsPathState host;
+    memset(&host, 0, sizeof host);
+    host.t = s->t0;
+    host.end = s->t_end;
+    host.tol = s->tol;
+    host.dt_min = s->dt_min;
+    host.dt_max = s->dt_max;
+    host.dt = (s->dt_max + s->dt_min) * 0.5;  // adams.rs:288
+    host.max_steps = 10000000ull;
+    for (uint32_t k = 0; k < s->dim; ++k) host.y[k] = s->y0[k];
+    for (size_t k = 0; k < s->params.size() && k < 8; ++k) host.p[k] = s->params[k];
+    return drain_path(s, host, [&](AdamsPathState *st, double *pt, double *py, uint32_t cap) {
+        return adams_launch_path(s->method, s->rhs, s->semantics, s->dim, st, pt, py, cap,
+                                 nullptr);
+    });
 }
 
 extern "C" int bg_solve(bg_solver *s) {
     if (!s) return BG_ERR_INVALID_ARGUMENT;
-    // rk.rs:250-256 / bdf.rs:258-264
+    // rk.rs:250-256 / bdf.rs:258-264 / adams.rs:250-256
     if (!s->have_dt_max || !s->have_dt_min || !s->have_tol || !s->have_t0 || !s->have_t_end ||
         !s->have_y0 || !s->have_deriv)
         return BG_MISSING_PARAMETERS;
     s->solved = false;
     s->fused = false;
     s->next = 0;
-    // Adams runs batched only (bg_ensemble_*, bg_adams_solve_device)
-    if (is_adams(s->method)) return BG_ERR_UNSUPPORTED;
-    int rc = is_rk(s->method) ? solve_rk_path(s) : solve_bdf_path(s);
+    const int rc = is_rk(s->method)      ? solve_rk_path(s)
+                   : is_adams(s->method) ? solve_adams_path(s)
+                                         : solve_bdf_path(s);
     if (rc != BG_OK) return rc;
     s->solved = true;
     return BG_OK;

@@ -28,6 +28,21 @@ int rk_launch_path_ensemble(int method, int rhs, int semantics, const bg_ensembl
 bool adams_supported(int method, int rhs, uint32_t dim);
 int adams_launch_ensemble(int method, int rhs, int semantics, const bg_ensemble_desc &d,
                           hipStream_t stream, void *ws = nullptr);
+// Single-instance Adams path state (resumable, like PathState below); the
+// Adams right-hand sides have at most 3 components.
+struct AdamsPathState {
+    double t, dt, end, tol, dt_min, dt_max;
+    uint64_t steps, max_steps;
+    uint32_t n_accepted, n_rejected;
+    int32_t vlen, ym, status, finished;
+    uint32_t count, pad;
+    double y[4], save[4], impl[4];
+    double dq[4][4];
+    double vt[4], vy[4][4];
+    double p[8];
+};
+int adams_launch_path(int method, int rhs, int semantics, uint32_t dim, AdamsPathState *state,
+                      double *path_t, double *path_y, uint32_t cap, hipStream_t stream);
 
 // Autotuner (tune.hip)
 int rk_autotune(int method, int rhs, int semantics, const bg_ensemble_desc &d, bg_tuning &tn,

@@ -153,14 +153,18 @@ __global__ void __launch_bounds__(256)
                 for (int ks = 0; ks < LN / 4; ++ks) {
                     const double bop = ytile[4 * ks + (l >> 4)][l & 15];
                     acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(aop[0][ks], bop, acc0, 0, 0, 0);
+                }
+#pragma unroll
+                for (int ks = 0; ks < LN / 4; ++ks) {
+                    const double bop = ytile[4 * ks + (l >> 4)][l & 15];
                     acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(aop[1][ks], bop, acc1, 0, 0, 0);
                 }
+                // the first M-tile's scaling overlaps the second chain's MFMAs
+#pragma unroll
+                for (int r = 0; r < 4; ++r) K[i][0][r] = (acc0[r] + bl[0][r]) * dt;
                 __syncthreads();
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    K[i][0][r] = (acc0[r] + bl[0][r]) * dt;
-                    K[i][1][r] = (acc1[r] + bl[1][r]) * dt;
-                }
+                for (int r = 0; r < 4; ++r) K[i][1][r] = (acc1[r] + bl[1][r]) * dt;
             }
             // ---- error (rk.rs:386-390), squared, reduced over the column
             constexpr int e0 = Tab::e_first();

@@ -199,6 +199,16 @@ class BDF2(_Solver):
     METHOD = L.BDF2
 
 
+class Adams5(_Solver):
+    """Adams5 (adams.rs:633)."""
+    METHOD = L.ADAMS5
+
+
+class Adams3(_Solver):
+    """Adams3 (adams.rs:693)."""
+    METHOD = L.ADAMS3
+
+
 RK45 = RungeKutta45  # README.md:23-47 spelling
 
 

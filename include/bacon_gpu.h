@@ -54,8 +54,8 @@ extern "C" {
 #define BG_RK23 1 /* RungeKutta23, rk.rs:656 */
 #define BG_BDF6 2 /* BDF6, bdf.rs:706        */
 #define BG_BDF2 3 /* BDF2, bdf.rs:762        */
-#define BG_ADAMS5 4 /* Adams5, adams.rs:633  (ensembles)  */
-#define BG_ADAMS3 5 /* Adams3, adams.rs:693  (ensembles)  */
+#define BG_ADAMS5 4 /* Adams5, adams.rs:633 */
+#define BG_ADAMS3 5 /* Adams3, adams.rs:693 */
 
 /* ---- semantics: the reference's quirk register as data ------------------
  * BG_AS_BUILT : the shipped behaviour (RK stage matrix read transposed,
@@ -116,8 +116,9 @@ int bg_with_dt_min(bg_solver *s, double min);
 int bg_with_dt_max(bg_solver *s, double max);
 int bg_with_start(bg_solver *s, double initial);
 int bg_with_end(bg_solver *s, double ending);
-/* solve(data) (rk.rs:249-343): checks MissingParameters, then produces the
- * path on the GPU. */
+/* solve(data) (rk.rs:249-343, bdf.rs:257-331, adams.rs:249-333): checks
+ * MissingParameters, then produces the path on the GPU (one lane; for Adams
+ * the iterator's yields, bootstrap replays of adams.rs:415-428 included). */
 int bg_solve(bg_solver *s);
 /* IVPIterator::next (ivp.rs:220-238): BG_OK with (t, y[dim]) filled,
  * BG_DONE at the end, or the failure code once (then BG_DONE: fused). */

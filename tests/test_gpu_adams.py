@@ -69,18 +69,80 @@ def test_adams_reference_problems(gpu, orc, method, method_o):
             _check(w, g, method_o, sem_o, [0, 63])
 
 
-def test_adams_single_instance_path_unsupported(gpu):
-    s = ivp.Solver(L.ADAMS5, 1) if hasattr(ivp, "Solver") else None
-    import ctypes as C
-    h = C.c_void_p()
-    lib = L.lib()
-    assert lib.bg_new(L.ADAMS5, L.DIM_STATIC, 1, C.byref(h)) == 0
-    for fn, v in (("bg_with_tolerance", 1e-4), ("bg_with_minimum_dt", 1e-5),
-                  ("bg_with_maximum_dt", 0.1), ("bg_with_initial_time", 0.0),
-                  ("bg_with_ending_time", 1.0)):
-        assert getattr(lib, fn)(h, v) == 0
-    y0 = (C.c_double * 1)(1.0)
-    assert lib.bg_with_initial_conditions_slice(h, y0, 1) == 0
-    assert lib.bg_with_derivative(h, L.RHS["exp"], None, 0) == 0
-    assert lib.bg_solve(h) == -3
-    lib.bg_destroy(h)
+def _oracle_path(method_o, rhs, y0, t_end, tol, dt_min, dt_max, sem_o, params=None, cap=400000):
+    return O.solve(method_o, rhs, y0, 0.0, t_end, tol, dt_min, dt_max, params=params,
+                   semantics=sem_o, profile=O.PROFILE_DEVICE, path_cap=cap)
+
+
+def _assert_path(path, r):
+    assert len(path) == r["path_len"]
+    assert np.array_equal(np.array([p[0] for p in path]), r["path_t"])
+    assert np.array_equal(np.array([p[1] for p in path]).reshape(r["path_y"].shape), r["path_y"])
+
+
+@pytest.mark.parametrize("sem,sem_o", SEM)
+@pytest.mark.parametrize("cls", [ivp.Adams5, ivp.Adams3])
+def test_single_instance_reference_problems(gpu, orc, cls, sem, sem_o):
+    """adams.rs:713-922 through the single-instance builder and iterator:
+    every yield (the bootstrap replays of adams.rs:415-428 included) equals
+    the oracle's path bit for bit and stays on the exact solution."""
+    method_o = O.ADAMS5 if cls is ivp.Adams5 else O.ADAMS3
+    a5 = cls is ivp.Adams5
+    cases = [("exp", 1.0, 2.0, 1e-5, 0.1, 5e-4 if a5 else 1e-3, np.exp, 0.01),
+             ("quadratic", 1.0, 5.0, 1e-7 if a5 else 1e-5, 0.001, 0.01 if a5 else 0.1,
+              lambda t: 1.0 - t * t, 0.01)]
+    for rhs, y0, t_end, dt_min, dt_max, tol, exact, bar in cases:
+        s = (cls.new(1).with_minimum_dt(dt_min).with_maximum_dt(dt_max).with_tolerance(tol)
+             .with_initial_time(0.0).with_ending_time(t_end)
+             .with_initial_conditions_slice([y0]).with_derivative(rhs).with_semantics(sem))
+        path = s.solve().collect_vec()
+        assert len(path) > 0
+        for t, y in path:
+            assert abs(y[0] - exact(t)) <= bar * max(1.0, abs(exact(t)))
+        _assert_path(path, _oracle_path(method_o, rhs, [y0], t_end, tol, dt_min, dt_max, sem_o))
+
+
+def test_single_instance_lorenz_and_as_built_failure(gpu, orc):
+    """Adams5 on one Lorenz-63 instance: intended runs to the end; as built
+    the stale derivative history ends the path with MinimumTimeDeltaExceeded,
+    after the same yields as the oracle's, and the iterator is fused."""
+    w = W.lorenz_plain(1)
+    y0, p = w["y0"][:, 0], w["params"][:, 0]
+
+    def mk(sem):
+        return (ivp.Adams5.new(3).with_minimum_dt(1e-6).with_maximum_dt(0.1)
+                .with_tolerance(1e-6).with_initial_time(0.0).with_ending_time(10.0)
+                .with_initial_conditions(y0).with_derivative("lorenz", p).with_semantics(sem))
+    path = mk(L.INTENDED).solve().collect_vec()
+    r = _oracle_path(O.ADAMS5, "lorenz", y0, 10.0, 1e-6, 1e-6, 0.1, O.INTENDED, params=p)
+    assert r["status"] == 0 and path[-1][0] == 10.0
+    _assert_path(path, r)
+    it = mk(L.AS_BUILT).solve()
+    vals = []
+    with pytest.raises(ivp.IVPError) as e:
+        for v in it:
+            vals.append(v)
+    assert e.value.name == "MinimumTimeDeltaExceeded"
+    assert list(it) == []
+    r = _oracle_path(O.ADAMS5, "lorenz", y0, 10.0, 1e-6, 1e-6, 0.1, O.AS_BUILT, params=p)
+    assert r["status"] == O.MINIMUM_TIME_DELTA_EXCEEDED
+    _assert_path(vals, r)
+
+
+def test_single_instance_path_resumes_across_buffers(gpu, orc):
+    """More yields than one launch's path buffer (65,536): the lane state is
+    saved and the kernel relaunched; the joined path equals the oracle's."""
+    s = (ivp.Adams3.new(1).with_minimum_dt(1e-6).with_maximum_dt(1e-3).with_tolerance(1e-3)
+         .with_initial_time(0.0).with_ending_time(100.0)
+         .with_initial_conditions_slice([1.0]).with_derivative("neg"))
+    path = s.solve().collect_vec()
+    assert len(path) > 70000
+    _assert_path(path, _oracle_path(O.ADAMS3, "neg", [1.0], 100.0, 1e-3, 1e-6, 1e-3,
+                                    O.INTENDED))
+
+
+def test_single_instance_missing_parameters(gpu):
+    s = ivp.Adams5.new(1).with_tolerance(1e-4).with_minimum_dt(1e-5)
+    with pytest.raises(ivp.IVPError) as e:
+        s.solve()
+    assert e.value.name == "MissingParameters"
