@@ -15,8 +15,8 @@
 //    each observation it stores the residual y(t_k) - target_k.
 //  * backward: segments newest first.  A segment re-runs the adaptive solve
 //    from its checkpoint (bit-identical to the forward: same operations),
-//    recording its K accepted steps (t, dt, stop, y) in a per-lane buffer,
-//    then sweeps them in reverse: lambda += 2 r_k at an observation, and the
+//    recording its K accepted steps (t, dt, stop flag, y) on a per-lane tape
+//    in LDS, then sweeps them in reverse: lambda += 2 r_k at an observation, and the
 //    stage-by-stage VJP of the step map with the right-hand side's
 //    hand-written Jacobian-vector products.
 // Storage layouts are structure-of-arrays with the instance (or lane slot) as
@@ -34,7 +34,7 @@ struct AdjWs {
     double *ck;      // [maxck][F][n], F = 3 + D: t, dt, next_stop, y[D]
     uint32_t *nck;   // [n] checkpoints written
     double *res;     // [n_obs][D][n] residuals y(t_k) - target_k
-    double *seg;     // [K][F][slots] backward segment records
+    double *seg;     // [K][2 + D][slots] backward tape, when not in LDS
     uint64_t slots;
     uint32_t *counters;  // [0] forward queue, [1] backward queue
     const int32_t *order_b;  // [n] backward schedule: most checkpoints first
@@ -249,15 +249,24 @@ __device__ __forceinline__ void vjp_step(double t, double dt, const double (&y)[
     for (int c = 0; c < D; ++c) lam[c] += ysum[c];
 }
 
-template <class RHS, class Tab>
+// The backward pass's tape: per lane, the K accepted steps of the segment
+// being reversed, FT = 2 + D doubles each: t, dt (negated when the step
+// landed on an observation time; dt > 0 or +0, so the sign bit is free and
+// negation is exact), y.  LDS_TAPE: in LDS, [K][FT][blockDim] (one block of
+// 256 lanes holds 128 KiB for Lotka-Volterra at K = 16), else in HBM with
+// the lane slot as the fastest index.
+template <class RHS, class Tab, bool LDS_TAPE>
 __global__ void __launch_bounds__(256)
     rk_adjoint_backward_kernel(const bg_adjoint_desc *__restrict__ ad, const AdjWs ws) {
-    constexpr int D = RHS::D, F = 3 + D, P = RHS::P > 0 ? RHS::P : 1;
+    constexpr int D = RHS::D, F = 3 + D, FT = 2 + D, P = RHS::P > 0 ? RHS::P : 1;
+    extern __shared__ double tape_lds[];
     r8_init();
     const bg_ensemble_desc &d = ad->base;
     const int lane = __lane_id();
     const uint64_t slot = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t n = d.n, S = ws.slots;
+    const uint64_t n = d.n;
+    double *const tape = LDS_TAPE ? tape_lds + threadIdx.x : ws.seg + slot;
+    const uint64_t S = LDS_TAPE ? (uint64_t)blockDim.x : ws.slots;
     const uint32_t K = ad->checkpoint_every ? ad->checkpoint_every : 16u;
     const uint32_t n_obs = ad->n_obs;
     const double *obs = ad->obs_t;
@@ -326,37 +335,58 @@ __global__ void __launch_bounds__(256)
                         bool clamped;
                         const int r = L.attempt_to(target, clamped);
                         if (r == STEP_ACCEPTED) {
-                            double stop_idx = -1.0;
-                            if (clamped && to_stop) {
+                            const bool at_stop = clamped && to_stop;
+                            if (at_stop) {
                                 L.t = target;
-                                stop_idx = (double)next_stop;
                                 ++next_stop;
                             }
-                            double *rec = ws.seg + ((uint64_t)cnt * F) * S + slot;
+                            double *rec = tape + ((uint64_t)cnt * FT) * S;
                             rec[0 * S] = t_before;
-                            rec[1 * S] = dt_used;
-                            rec[2 * S] = stop_idx;
+                            rec[1 * S] = at_stop ? -dt_used : dt_used;
 #pragma unroll
-                            for (int c = 0; c < D; ++c) rec[(3 + c) * S] = y_before[c];
+                            for (int c = 0; c < D; ++c) rec[(2 + c) * S] = y_before[c];
                             ++cnt;
                         } else if (r != STEP_REJECTED) {
                             break;  // cannot happen on a replay of a successful forward
                         }
                     }
                     // ---- reverse sweep ----
-                    for (int k = (int)cnt - 1; k >= 0; --k) {
-                        const double *rec = ws.seg + ((uint64_t)k * F) * S + slot;
-                        const double tk = rec[0 * S], dk = rec[1 * S], sk = rec[2 * S];
-                        double yk[D];
+                    // Observation steps take the stop indices in reverse:
+                    // the replay ended with next_stop one past the last.
+                    // Step k - 1's record is loaded while step k reverses.
+                    uint32_t cursor = next_stop;
+                    double tk = 0.0, dk = 0.0, yk[D];
 #pragma unroll
-                        for (int c = 0; c < D; ++c) yk[c] = rec[(3 + c) * S];
-                        if (sk >= 0.0) {
-                            const uint64_t s = (uint64_t)sk;
+                    for (int c = 0; c < D; ++c) yk[c] = 0.0;
+                    if (cnt > 0) {
+                        const double *rec = tape + ((uint64_t)(cnt - 1) * FT) * S;
+                        tk = rec[0 * S];
+                        dk = rec[1 * S];
+#pragma unroll
+                        for (int c = 0; c < D; ++c) yk[c] = rec[(2 + c) * S];
+                    }
+                    for (int k = (int)cnt - 1; k >= 0; --k) {
+                        double tn = 0.0, dn = 0.0, yn[D];
+#pragma unroll
+                        for (int c = 0; c < D; ++c) yn[c] = 0.0;
+                        if (k > 0) {
+                            const double *rec = tape + ((uint64_t)(k - 1) * FT) * S;
+                            tn = rec[0 * S];
+                            dn = rec[1 * S];
+#pragma unroll
+                            for (int c = 0; c < D; ++c) yn[c] = rec[(2 + c) * S];
+                        }
+                        if (signbit(dk)) {
+                            const uint64_t s = --cursor;
 #pragma unroll
                             for (int c = 0; c < D; ++c)
                                 lam[c] = fma(2.0, ws.res[(s * D + c) * n + i], lam[c]);
                         }
-                        vjp_step<RHS, Tab>(tk, dk, yk, L.p, lam, pb);
+                        vjp_step<RHS, Tab>(tk, fabs(dk), yk, L.p, lam, pb);
+                        tk = tn;
+                        dk = dn;
+#pragma unroll
+                        for (int c = 0; c < D; ++c) yk[c] = yn[c];
                     }
                     --seg;
                     if (seg < 0) fin = true;
@@ -400,7 +430,7 @@ int rk_launch_adjoint(int method, int rhs, const bg_adjoint_desc &ad, hipStream_
     auto run = [&](auto rtag) -> int {
         using R = typename decltype(rtag)::type;
         using T = Tableau<Fehlberg45, false>;
-        constexpr int D = R::D, F = 3 + D;
+        constexpr int D = R::D, F = 3 + D, FT = 2 + D;
         if (d.dim != (uint32_t)D) return BG_ERR_INVALID_ARGUMENT;
         if (d.n == 0) return BG_OK;
         if (d.n > 0x7fffffffull || !d.y0 || !d.params || d.n_params < (uint32_t)R::P ||
@@ -412,31 +442,46 @@ int rk_launch_adjoint(int method, int rhs, const bg_adjoint_desc &ad, hipStream_
         int dev = 0;
         BG_HIP_CHECK(hipGetDevice(&dev));
         const int cus = device_cu_count(dev);
+        // The backward tape goes to LDS when a 256-lane block's fits (K = 16:
+        // Lotka-Volterra 128 KiB); larger tapes stay in HBM.
+        int lds_max = 0;
+        BG_HIP_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        const size_t tape_bytes = (size_t)K * FT * sizeof(double) * 256;
+        const bool lds_tape = tape_bytes + 1024 <= (size_t)lds_max;
+        auto bwd = lds_tape ? rk_adjoint_backward_kernel<R, T, true>
+                            : rk_adjoint_backward_kernel<R, T, false>;
+        const size_t dyn_lds = lds_tape ? tape_bytes : 0;
+        if (lds_tape)
+            BG_HIP_CHECK(hipFuncSetAttribute((const void *)bwd,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)dyn_lds));
         int per_cu_f = 0, per_cu_b = 0;
         BG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu_f, rk_adjoint_forward_kernel<R, T>, 256, 0));
-        BG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu_b, rk_adjoint_backward_kernel<R, T>, 256, 0));
+        BG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_b, bwd, 256, dyn_lds));
         per_cu_f = per_cu_f < 1 ? 1 : per_cu_f;
         per_cu_b = per_cu_b < 1 ? 1 : per_cu_b;
-        // fewer resident lanes than the occupancy limit: each lane works
+        // Fewer resident lanes than the occupancy limit: each lane works
         // through several instances in schedule order, so lanes finish
-        // together (measured best at 2 blocks per CU for both passes)
-        const int want = ad.blocks_per_cu ? (int)ad.blocks_per_cu : 2;
-        if (per_cu_f > want) per_cu_f = want;
-        if (per_cu_b > want) per_cu_b = want;
+        // together.  Measured on Lotka-Volterra: forward best at 2 blocks
+        // per CU; backward best at 1 (with an HBM tape the tape's L2
+        // footprint grows with the resident lanes).
+        const int want_f = ad.blocks_per_cu ? (int)ad.blocks_per_cu : 2;
+        const int want_b = ad.blocks_per_cu ? (int)ad.blocks_per_cu : 1;
+        if (per_cu_f > want_f) per_cu_f = want_f;
+        if (per_cu_b > want_b) per_cu_b = want_b;
         uint64_t grid_f = (uint64_t)per_cu_f * cus, grid_b = (uint64_t)per_cu_b * cus;
         const uint64_t need = (n + 255) / 256;
         if (grid_f > need) grid_f = need;
         if (grid_b > need) grid_b = need;
-        const uint64_t slots = grid_b * 256;
+        const uint64_t slots = lds_tape ? 0 : grid_b * 256;
         const uint64_t n_res = (uint64_t)(ad.n_obs ? ad.n_obs : 1) * D * n;
         size_t sort_bytes = 0;
         BG_HIP_CHECK(rocprim::radix_sort_pairs_desc(
             nullptr, sort_bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
             (const int32_t *)nullptr, (int32_t *)nullptr, (uint32_t)n, 0, 32, stream));
         const size_t bytes = 256 + sizeof(bg_adjoint_desc) + 64 +
-                             sizeof(double) * (maxck * F * n + n_res + K * F * slots) +
+                             sizeof(double) * (maxck * F * n + n_res + K * FT * slots) +
                              sizeof(uint32_t) * n + 256 + 4 * (sizeof(uint32_t) * n + 256) +
                              sort_bytes;
         char *buf = nullptr;
@@ -449,7 +494,7 @@ int rk_launch_adjoint(int method, int rhs, const bg_adjoint_desc &ad, hipStream_
         ws.ck = (double *)p;
         ws.res = ws.ck + maxck * F * n;
         ws.seg = ws.res + n_res;
-        ws.nck = (uint32_t *)(ws.seg + K * F * slots);
+        ws.nck = (uint32_t *)(ws.seg + K * FT * slots);
         ws.slots = slots;
         auto align = [](char *q) { return q + (256 - ((uintptr_t)q % 256)) % 256; };
         char *q = align((char *)(ws.nck + n));
@@ -468,7 +513,7 @@ int rk_launch_adjoint(int method, int rhs, const bg_adjoint_desc &ad, hipStream_
         adj_order_keys_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, stream>>>(ws.nck, n, keys, idx);
         BG_HIP_CHECK(rocprim::radix_sort_pairs_desc(sort_tmp, sort_bytes, keys, keys_sorted, idx,
                                                     order_b, (uint32_t)n, 0, 32, stream));
-        rk_adjoint_backward_kernel<R, T><<<(uint32_t)grid_b, 256, 0, stream>>>(dd, ws);
+        bwd<<<(uint32_t)grid_b, 256, dyn_lds, stream>>>(dd, ws);
         BG_HIP_CHECK(hipGetLastError());
         BG_HIP_CHECK(hipFreeAsync(buf, stream));
         return BG_OK;
