@@ -148,3 +148,16 @@ def test_checkpoint_capacity_exceeded(gpu, lv):
     g = _gpu(lv, max_checkpoints=2)
     assert np.all(g["status"] == 9)
     assert np.all(np.isnan(g["grad_params"]))
+
+
+def test_tape_in_lds_and_hbm_agree(gpu, lv):
+    """K = 16 keeps the backward pass's tape in LDS (256 lanes x 16 steps x
+    4 doubles = 128 KiB); K = 48 does not fit and takes the HBM tape; K = 5
+    gives short, ragged segments.  The tape holds the forward's own steps in
+    every case, so the gradients are bit-identical."""
+    base = _gpu(lv)
+    for k, cap in ((5, 512), (48, 128)):
+        g = _gpu(lv, checkpoint_every=k, max_checkpoints=cap)
+        assert np.all(g["status"] == 0)
+        for key in ("grad_params", "grad_y0", "loss", "obs_y"):
+            assert np.array_equal(g[key], base[key]), (k, key)
