@@ -31,9 +31,9 @@ namespace bg {
 static constexpr uint32_t kAdjMaxAttempts = 10000000u;
 
 struct AdjWs {
-    double *ck;      // [maxck][F][n], F = 3 + D: t, dt, next_stop, y[D]
+    double *ck;      // [n][maxck][F], F = 3 + D: t, dt, next_stop, y[D]
     uint32_t *nck;   // [n] checkpoints written
-    double *res;     // [n_obs][D][n] residuals y(t_k) - target_k
+    double *res;     // [n][n_obs][D] residuals y(t_k) - target_k
     double *seg;     // [K][2 + D][slots] backward tape, when not in LDS
     uint64_t slots;
     uint32_t *counters;  // [0] forward queue, [1] backward queue
@@ -50,6 +50,25 @@ __global__ void adj_order_keys_kernel(const uint32_t *__restrict__ nck, uint64_t
     idx[i] = (int32_t)i;
 }
 
+// Observation times are read at every instance start (count_stops) and at
+// every stop.  With OBS_LDS (n_obs <= kObsLds) they are staged in LDS at
+// kernel entry: the reads then neither wait on L2 (a serial scan of 20
+// global loads per instance start cost microseconds) nor, being known LDS
+// accesses, wait for the loop's outstanding checkpoint stores.  Every thread
+// of the block calls it.
+static constexpr uint32_t kObsLds = 1024;
+template <bool OBS_LDS>
+__device__ __forceinline__ const double *stage_obs(double *lds, const double *obs,
+                                                   uint32_t n_obs) {
+    if constexpr (!OBS_LDS) {
+        return obs;
+    } else {
+        for (uint32_t k = threadIdx.x; k < n_obs; k += blockDim.x) lds[k] = obs[k];
+        __syncthreads();
+        return lds;
+    }
+}
+
 // Number of observation times strictly before `end` (they become stops).
 __device__ __forceinline__ uint32_t count_stops(const double *obs, uint32_t n_obs, double end) {
     uint32_t k = 0;
@@ -57,7 +76,7 @@ __device__ __forceinline__ uint32_t count_stops(const double *obs, uint32_t n_ob
     return k;
 }
 
-template <class RHS, class Tab>
+template <class RHS, class Tab, bool OBS_LDS>
 __global__ void __launch_bounds__(256)
     rk_adjoint_forward_kernel(const bg_adjoint_desc *__restrict__ ad, const AdjWs ws) {
     constexpr int D = RHS::D, F = 3 + D;
@@ -68,11 +87,13 @@ __global__ void __launch_bounds__(256)
     const uint32_t K = ad->checkpoint_every ? ad->checkpoint_every : 16u;
     const uint32_t maxck = ad->max_checkpoints ? ad->max_checkpoints : 128u;
     const uint32_t n_obs = ad->n_obs;
-    const double *obs = ad->obs_t;
+    __shared__ double obs_lds[OBS_LDS ? kObsLds : 1];
+    const double *obs = stage_obs<OBS_LDS>(obs_lds, ad->obs_t, n_obs);
     RkLane<RHS, Tab> L;
     int64_t inst = -1;
     uint32_t attempts = 0, nacc = 0, nrej = 0, next_stop = 0, n_stops = 0, nck = 0;
-    double loss = 0.0;
+    uint32_t ck_left = K;  // accepted steps until the next checkpoint
+    double loss = 0.0, target = 0.0;  // target: the next stop, else the end
     for (;;) {
         const uint64_t need = __ballot(inst == -1);
         if (need != 0) {
@@ -96,14 +117,17 @@ __global__ void __launch_bounds__(256)
                     L.dt_max = d.dt_max ? d.dt_max[i] : d.dt_max_s;
                     L.dt = (L.dt_max + L.dt_min) * 0.5;
                     attempts = nacc = nrej = next_stop = 0;
+                    ck_left = K;
                     n_stops = count_stops(obs, n_obs, L.end);
+                    target = n_stops > 0 ? obs[0] : L.end;
                     loss = 0.0;
                     // checkpoint 0: the initial state
-                    ws.ck[(0 * F + 0) * n + i] = L.t;
-                    ws.ck[(0 * F + 1) * n + i] = L.dt;
-                    ws.ck[(0 * F + 2) * n + i] = 0.0;
+                    double *ck0 = ws.ck + i * maxck * F;
+                    ck0[0] = L.t;
+                    ck0[1] = L.dt;
+                    ck0[2] = 0.0;
 #pragma unroll
-                    for (int c = 0; c < D; ++c) ws.ck[(0 * F + 3 + c) * n + i] = L.y[c];
+                    for (int c = 0; c < D; ++c) ck0[3 + c] = L.y[c];
                     nck = 1;
                 } else {
                     inst = -2;
@@ -121,7 +145,6 @@ __global__ void __launch_bounds__(256)
                     st = BG_MAXIMUM_ITERATIONS_EXCEEDED;
                 } else {
                     const bool to_stop = next_stop < n_stops;
-                    const double target = to_stop ? obs[next_stop] : L.end;
                     bool clamped;
                     const int r = L.attempt_to(target, clamped);
                     ++attempts;
@@ -134,24 +157,27 @@ __global__ void __launch_bounds__(256)
                             for (int c = 0; c < D; ++c) {
                                 const uint64_t o = ((uint64_t)next_stop * D + c) * n + i;
                                 const double rr = L.y[c] - ad->targets[o];
-                                ws.res[o] = rr;
+                                ws.res[(i * n_obs + next_stop) * D + c] = rr;
                                 loss = fma(rr, rr, loss);
                                 if (ad->obs_y) ad->obs_y[o] = L.y[c];
                             }
                             ++next_stop;
+                            target = next_stop < n_stops ? obs[next_stop] : L.end;
                         }
+                        --ck_left;
                         if (L.t >= L.end) {
                             st = BG_OK;
-                        } else if (nacc % K == 0) {
+                        } else if (ck_left == 0) {
+                            ck_left = K;
                             if (nck >= maxck) {
                                 st = BG_MAXIMUM_ITERATIONS_EXCEEDED;
                             } else {
-                                ws.ck[((uint64_t)nck * F + 0) * n + i] = L.t;
-                                ws.ck[((uint64_t)nck * F + 1) * n + i] = L.dt;
-                                ws.ck[((uint64_t)nck * F + 2) * n + i] = (double)next_stop;
+                                double *ckp = ws.ck + (i * maxck + nck) * F;
+                                ckp[0] = L.t;
+                                ckp[1] = L.dt;
+                                ckp[2] = (double)next_stop;
 #pragma unroll
-                                for (int c = 0; c < D; ++c)
-                                    ws.ck[((uint64_t)nck * F + 3 + c) * n + i] = L.y[c];
+                                for (int c = 0; c < D; ++c) ckp[3 + c] = L.y[c];
                                 ++nck;
                             }
                         }
@@ -172,7 +198,7 @@ __global__ void __launch_bounds__(256)
                 for (int c = 0; c < D; ++c) {
                     const uint64_t o = ((uint64_t)n_stops * D + c) * n + i;
                     const double rr = L.y[c] - ad->targets[o];
-                    ws.res[o] = rr;
+                    ws.res[(i * n_obs + n_stops) * D + c] = rr;
                     loss = fma(rr, rr, loss);
                     if (ad->obs_y) ad->obs_y[o] = L.y[c];
                 }
@@ -255,7 +281,7 @@ __device__ __forceinline__ void vjp_step(double t, double dt, const double (&y)[
 // negation is exact), y.  LDS_TAPE: in LDS, [K][FT][blockDim] (one block of
 // 256 lanes holds 128 KiB for Lotka-Volterra at K = 16), else in HBM with
 // the lane slot as the fastest index.
-template <class RHS, class Tab, bool LDS_TAPE>
+template <class RHS, class Tab, bool LDS_TAPE, bool OBS_LDS>
 __global__ void __launch_bounds__(256)
     rk_adjoint_backward_kernel(const bg_adjoint_desc *__restrict__ ad, const AdjWs ws) {
     constexpr int D = RHS::D, F = 3 + D, FT = 2 + D, P = RHS::P > 0 ? RHS::P : 1;
@@ -269,11 +295,23 @@ __global__ void __launch_bounds__(256)
     const uint64_t S = LDS_TAPE ? (uint64_t)blockDim.x : ws.slots;
     const uint32_t K = ad->checkpoint_every ? ad->checkpoint_every : 16u;
     const uint32_t n_obs = ad->n_obs;
-    const double *obs = ad->obs_t;
+    const uint64_t maxck = ad->max_checkpoints ? ad->max_checkpoints : 128u;
+    __shared__ double obs_lds[OBS_LDS ? kObsLds : 1];
+    const double *obs = stage_obs<OBS_LDS>(obs_lds, ad->obs_t, n_obs);
     RkLane<RHS, Tab> L;
     double lam[D], pb[P];
     int64_t inst = -1;
     int seg = -1;
+    uint32_t n_stops = 0;  // observation times before the instance's end
+    // checkpoint of segment `seg` (t, dt, next stop, y), loaded one segment
+    // ahead so the loads overlap the previous segment's work
+    double ck[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) ck[f] = 0.0;
+    auto load_ck = [&](int sg, uint64_t i) {
+#pragma unroll
+        for (int f = 0; f < F; ++f) ck[f] = ws.ck[(i * maxck + (uint64_t)sg) * F + f];
+    };
     for (;;) {
         const uint64_t need = __ballot(inst == -1);
         if (need != 0) {
@@ -294,13 +332,14 @@ __global__ void __launch_bounds__(256)
                     L.dt_max = d.dt_max ? d.dt_max[i] : d.dt_max_s;
 #pragma unroll
                     for (int c = 0; c < P; ++c) pb[c] = 0.0;
-                    const uint32_t ns = count_stops(obs, n_obs, L.end);
-                    const bool final_obs = ns < n_obs && obs[ns] == L.end;
+                    n_stops = count_stops(obs, n_obs, L.end);
+                    const bool final_obs = n_stops < n_obs && obs[n_stops] == L.end;
 #pragma unroll
                     for (int c = 0; c < D; ++c)
-                        lam[c] = final_obs ? 2.0 * ws.res[((uint64_t)ns * D + c) * n + i] : 0.0;
+                        lam[c] = final_obs ? 2.0 * ws.res[(i * n_obs + n_stops) * D + c] : 0.0;
                     const int fw = d.status ? d.status[i] : BG_OK;
                     seg = fw == BG_OK ? (int)ws.nck[i] - 1 : -1;
+                    if (seg >= 0) load_ck(seg, i);
                 } else {
                     inst = -2;
                 }
@@ -316,17 +355,17 @@ __global__ void __launch_bounds__(256)
                 } else {
                     const uint64_t i = (uint64_t)inst;
                     // ---- recompute segment `seg` from its checkpoint ----
-                    L.t = ws.ck[((uint64_t)seg * F + 0) * n + i];
-                    L.dt = ws.ck[((uint64_t)seg * F + 1) * n + i];
-                    uint32_t next_stop = (uint32_t)ws.ck[((uint64_t)seg * F + 2) * n + i];
+                    L.t = ck[0];
+                    L.dt = ck[1];
+                    uint32_t next_stop = (uint32_t)ck[2];
 #pragma unroll
-                    for (int c = 0; c < D; ++c) L.y[c] = ws.ck[((uint64_t)seg * F + 3 + c) * n + i];
-                    const uint32_t n_stops = count_stops(obs, n_obs, L.end);
+                    for (int c = 0; c < D; ++c) L.y[c] = ck[3 + c];
+                    if (seg > 0) load_ck(seg - 1, i);
+                    double target = next_stop < n_stops ? obs[next_stop] : L.end;
                     uint32_t cnt = 0, guard = 0;
                     while (cnt < K && L.t < L.end && guard < kAdjMaxAttempts) {
                         ++guard;
                         const bool to_stop = next_stop < n_stops;
-                        const double target = to_stop ? obs[next_stop] : L.end;
                         const double t_before = L.t;
                         double y_before[D];
 #pragma unroll
@@ -339,6 +378,7 @@ __global__ void __launch_bounds__(256)
                             if (at_stop) {
                                 L.t = target;
                                 ++next_stop;
+                                target = next_stop < n_stops ? obs[next_stop] : L.end;
                             }
                             double *rec = tape + ((uint64_t)cnt * FT) * S;
                             rec[0 * S] = t_before;
@@ -380,7 +420,7 @@ __global__ void __launch_bounds__(256)
                             const uint64_t s = --cursor;
 #pragma unroll
                             for (int c = 0; c < D; ++c)
-                                lam[c] = fma(2.0, ws.res[(s * D + c) * n + i], lam[c]);
+                                lam[c] = fma(2.0, ws.res[(i * n_obs + s) * D + c], lam[c]);
                         }
                         vjp_step<RHS, Tab>(tk, fabs(dk), yk, L.p, lam, pb);
                         tk = tn;
@@ -447,17 +487,21 @@ int rk_launch_adjoint(int method, int rhs, const bg_adjoint_desc &ad, hipStream_
         int lds_max = 0;
         BG_HIP_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
         const size_t tape_bytes = (size_t)K * FT * sizeof(double) * 256;
-        const bool lds_tape = tape_bytes + 1024 <= (size_t)lds_max;
-        auto bwd = lds_tape ? rk_adjoint_backward_kernel<R, T, true>
-                            : rk_adjoint_backward_kernel<R, T, false>;
+        const bool lds_tape = tape_bytes + sizeof(double) * kObsLds + 1024 <= (size_t)lds_max;
+        const bool obs_lds = ad.n_obs <= kObsLds;
+        auto fwd = obs_lds ? rk_adjoint_forward_kernel<R, T, true>
+                           : rk_adjoint_forward_kernel<R, T, false>;
+        auto bwd = lds_tape ? (obs_lds ? rk_adjoint_backward_kernel<R, T, true, true>
+                                       : rk_adjoint_backward_kernel<R, T, true, false>)
+                            : (obs_lds ? rk_adjoint_backward_kernel<R, T, false, true>
+                                       : rk_adjoint_backward_kernel<R, T, false, false>);
         const size_t dyn_lds = lds_tape ? tape_bytes : 0;
         if (lds_tape)
             BG_HIP_CHECK(hipFuncSetAttribute((const void *)bwd,
                                              hipFuncAttributeMaxDynamicSharedMemorySize,
                                              (int)dyn_lds));
         int per_cu_f = 0, per_cu_b = 0;
-        BG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu_f, rk_adjoint_forward_kernel<R, T>, 256, 0));
+        BG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_f, fwd, 256, 0));
         BG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_b, bwd, 256, dyn_lds));
         per_cu_f = per_cu_f < 1 ? 1 : per_cu_f;
         per_cu_b = per_cu_b < 1 ? 1 : per_cu_b;
@@ -509,7 +553,7 @@ int rk_launch_adjoint(int method, int rhs, const bg_adjoint_desc &ad, hipStream_
         void *sort_tmp = q;
         ws.order_b = order_b;
         adj_setup_kernel<<<1, 64, 0, stream>>>(ws.counters, dd, ad);
-        rk_adjoint_forward_kernel<R, T><<<(uint32_t)grid_f, 256, 0, stream>>>(dd, ws);
+        fwd<<<(uint32_t)grid_f, 256, 0, stream>>>(dd, ws);
         adj_order_keys_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, stream>>>(ws.nck, n, keys, idx);
         BG_HIP_CHECK(rocprim::radix_sort_pairs_desc(sort_tmp, sort_bytes, keys, keys_sorted, idx,
                                                     order_b, (uint32_t)n, 0, 32, stream));
