@@ -94,6 +94,18 @@ __global__ void __launch_bounds__(256)
     uint32_t attempts = 0, nacc = 0, nrej = 0, next_stop = 0, n_stops = 0, nck = 0;
     uint32_t ck_left = K;  // accepted steps until the next checkpoint
     double loss = 0.0, target = 0.0;  // target: the next stop, else the end
+    // targets of observation next_stop, loaded when the previous observation
+    // is passed: in a wave some lane is at a stop on most attempts, and a
+    // load there would stall every lane
+    double tgt[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) tgt[c] = 0.0;
+    auto load_tgt = [&](uint32_t k, uint64_t i) {
+        if (k < n_obs) {
+#pragma unroll
+            for (int c = 0; c < D; ++c) tgt[c] = ad->targets[((uint64_t)k * D + c) * n + i];
+        }
+    };
     for (;;) {
         const uint64_t need = __ballot(inst == -1);
         if (need != 0) {
@@ -120,6 +132,7 @@ __global__ void __launch_bounds__(256)
                     ck_left = K;
                     n_stops = count_stops(obs, n_obs, L.end);
                     target = n_stops > 0 ? obs[0] : L.end;
+                    load_tgt(0, i);
                     loss = 0.0;
                     // checkpoint 0: the initial state
                     double *ck0 = ws.ck + i * maxck * F;
@@ -156,13 +169,14 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
                             for (int c = 0; c < D; ++c) {
                                 const uint64_t o = ((uint64_t)next_stop * D + c) * n + i;
-                                const double rr = L.y[c] - ad->targets[o];
+                                const double rr = L.y[c] - tgt[c];
                                 ws.res[(i * n_obs + next_stop) * D + c] = rr;
                                 loss = fma(rr, rr, loss);
                                 if (ad->obs_y) ad->obs_y[o] = L.y[c];
                             }
                             ++next_stop;
                             target = next_stop < n_stops ? obs[next_stop] : L.end;
+                            load_tgt(next_stop, i);
                         }
                         --ck_left;
                         if (L.t >= L.end) {
@@ -197,7 +211,8 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
                 for (int c = 0; c < D; ++c) {
                     const uint64_t o = ((uint64_t)n_stops * D + c) * n + i;
-                    const double rr = L.y[c] - ad->targets[o];
+                    // every stop was passed: tgt holds observation n_stops
+                    const double rr = L.y[c] - (next_stop == n_stops ? tgt[c] : ad->targets[o]);
                     ws.res[(i * n_obs + n_stops) * D + c] = rr;
                     loss = fma(rr, rr, loss);
                     if (ad->obs_y) ad->obs_y[o] = L.y[c];
@@ -312,6 +327,19 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int f = 0; f < F; ++f) ck[f] = ws.ck[(i * maxck + (uint64_t)sg) * F + f];
     };
+    // residual of the stop the reverse sweep meets next (stops are met in
+    // decreasing order, n_stops - 1 first), loaded one stop ahead
+    int res_k = -1;
+    double rn[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) rn[c] = 0.0;
+    auto load_res = [&](int k, uint64_t i) {
+        res_k = k;
+        if (k >= 0) {
+#pragma unroll
+            for (int c = 0; c < D; ++c) rn[c] = ws.res[(i * n_obs + (uint64_t)k) * D + c];
+        }
+    };
     for (;;) {
         const uint64_t need = __ballot(inst == -1);
         if (need != 0) {
@@ -339,6 +367,7 @@ __global__ void __launch_bounds__(256)
                         lam[c] = final_obs ? 2.0 * ws.res[(i * n_obs + n_stops) * D + c] : 0.0;
                     const int fw = d.status ? d.status[i] : BG_OK;
                     seg = fw == BG_OK ? (int)ws.nck[i] - 1 : -1;
+                    load_res((int)n_stops - 1, i);
                     if (seg >= 0) load_ck(seg, i);
                 } else {
                     inst = -2;
@@ -418,9 +447,10 @@ __global__ void __launch_bounds__(256)
                         }
                         if (signbit(dk)) {
                             const uint64_t s = --cursor;
+                            if ((int)s != res_k) load_res((int)s, i);
 #pragma unroll
-                            for (int c = 0; c < D; ++c)
-                                lam[c] = fma(2.0, ws.res[(i * n_obs + s) * D + c], lam[c]);
+                            for (int c = 0; c < D; ++c) lam[c] = fma(2.0, rn[c], lam[c]);
+                            load_res((int)s - 1, i);
                         }
                         vjp_step<RHS, Tab>(tk, fabs(dk), yk, L.p, lam, pb);
                         tk = tn;
