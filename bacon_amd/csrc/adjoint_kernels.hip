@@ -33,6 +33,7 @@ static constexpr uint32_t kAdjMaxAttempts = 10000000u;
 struct AdjWs {
     double *ck;      // [n][maxck][F], F = 3 + D: t, dt, next_stop, y[D]
     uint32_t *nck;   // [n] checkpoints written
+    uint32_t *nobs;  // [n] observations reached
     double *res;     // [n][n_obs][D] residuals y(t_k) - target_k
     double *seg;     // [K][2 + D][slots] backward tape, when not in LDS
     uint64_t slots;
@@ -63,7 +64,7 @@ __device__ __forceinline__ const double *stage_obs(double *lds, const double *ob
     if constexpr (!OBS_LDS) {
         return obs;
     } else {
-        for (uint32_t k = threadIdx.x; k < n_obs; k += blockDim.x) lds[k] = obs[k];
+        for (uint32_t k = threadIdx.x; k < n_obs; k += blockDim.x) lds[k] = gptr(obs)[k];
         __syncthreads();
         return lds;
     }
@@ -93,19 +94,7 @@ __global__ void __launch_bounds__(256)
     int64_t inst = -1;
     uint32_t attempts = 0, nacc = 0, nrej = 0, next_stop = 0, n_stops = 0, nck = 0;
     uint32_t ck_left = K;  // accepted steps until the next checkpoint
-    double loss = 0.0, target = 0.0;  // target: the next stop, else the end
-    // targets of observation next_stop, loaded when the previous observation
-    // is passed: in a wave some lane is at a stop on most attempts, and a
-    // load there would stall every lane
-    double tgt[D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) tgt[c] = 0.0;
-    auto load_tgt = [&](uint32_t k, uint64_t i) {
-        if (k < n_obs) {
-#pragma unroll
-            for (int c = 0; c < D; ++c) tgt[c] = ad->targets[((uint64_t)k * D + c) * n + i];
-        }
-    };
+    double target = 0.0;  // the next stop, else the end
     for (;;) {
         const uint64_t need = __ballot(inst == -1);
         if (need != 0) {
@@ -116,24 +105,22 @@ __global__ void __launch_bounds__(256)
             if (inst == -1) {
                 const uint64_t k = (uint64_t)base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
                 if (k < n) {
-                    inst = d.order ? (int64_t)d.order[k] : (int64_t)k;
+                    inst = d.order ? (int64_t)gptr(d.order)[k] : (int64_t)k;
                     const uint64_t i = (uint64_t)inst;
 #pragma unroll
-                    for (int c = 0; c < D; ++c) L.y[c] = d.y0[c * n + i];
+                    for (int c = 0; c < D; ++c) L.y[c] = gptr(d.y0)[c * n + i];
 #pragma unroll
-                    for (int c = 0; c < RHS::P; ++c) L.p[c] = d.params[c * n + i];
-                    L.t = d.t0 ? d.t0[i] : d.t0_s;
-                    L.end = d.t_end ? d.t_end[i] : d.t_end_s;
-                    L.tol = d.tol ? d.tol[i] : d.tol_s;
-                    L.dt_min = d.dt_min ? d.dt_min[i] : d.dt_min_s;
-                    L.dt_max = d.dt_max ? d.dt_max[i] : d.dt_max_s;
+                    for (int c = 0; c < RHS::P; ++c) L.p[c] = gptr(d.params)[c * n + i];
+                    L.t = d.t0 ? gptr(d.t0)[i] : d.t0_s;
+                    L.end = d.t_end ? gptr(d.t_end)[i] : d.t_end_s;
+                    L.tol = d.tol ? gptr(d.tol)[i] : d.tol_s;
+                    L.dt_min = d.dt_min ? gptr(d.dt_min)[i] : d.dt_min_s;
+                    L.dt_max = d.dt_max ? gptr(d.dt_max)[i] : d.dt_max_s;
                     L.dt = (L.dt_max + L.dt_min) * 0.5;
                     attempts = nacc = nrej = next_stop = 0;
                     ck_left = K;
                     n_stops = count_stops(obs, n_obs, L.end);
                     target = n_stops > 0 ? obs[0] : L.end;
-                    load_tgt(0, i);
-                    loss = 0.0;
                     // checkpoint 0: the initial state
                     double *ck0 = ws.ck + i * maxck * F;
                     ck0[0] = L.t;
@@ -167,16 +154,10 @@ __global__ void __launch_bounds__(256)
                         if (clamped && to_stop) {
                             L.t = target;
 #pragma unroll
-                            for (int c = 0; c < D; ++c) {
-                                const uint64_t o = ((uint64_t)next_stop * D + c) * n + i;
-                                const double rr = L.y[c] - tgt[c];
-                                ws.res[(i * n_obs + next_stop) * D + c] = rr;
-                                loss = fma(rr, rr, loss);
-                                if (ad->obs_y) ad->obs_y[o] = L.y[c];
-                            }
+                            for (int c = 0; c < D; ++c)
+                                ws.res[(i * n_obs + next_stop) * D + c] = L.y[c];
                             ++next_stop;
                             target = next_stop < n_stops ? obs[next_stop] : L.end;
-                            load_tgt(next_stop, i);
                         }
                         --ck_left;
                         if (L.t >= L.end) {
@@ -207,34 +188,55 @@ __global__ void __launch_bounds__(256)
             const uint64_t i = (uint64_t)inst;
             // an observation at the end time (after the last stop) scores the
             // final state
+            uint32_t reached = next_stop;
             if (st == BG_OK && n_stops < n_obs && obs[n_stops] == L.end) {
 #pragma unroll
-                for (int c = 0; c < D; ++c) {
-                    const uint64_t o = ((uint64_t)n_stops * D + c) * n + i;
-                    // every stop was passed: tgt holds observation n_stops
-                    const double rr = L.y[c] - (next_stop == n_stops ? tgt[c] : ad->targets[o]);
-                    ws.res[(i * n_obs + n_stops) * D + c] = rr;
-                    loss = fma(rr, rr, loss);
-                    if (ad->obs_y) ad->obs_y[o] = L.y[c];
-                }
+                for (int c = 0; c < D; ++c) ws.res[(i * n_obs + n_stops) * D + c] = L.y[c];
+                reached = n_stops + 1;
             }
             ws.nck[i] = nck;
-            if (ad->loss) ad->loss[i] = loss;
-            if (d.t_out) d.t_out[i] = L.t;
+            ws.nobs[i] = reached;
+            if (d.t_out) gptr(d.t_out)[i] = L.t;
             if (d.y_out) {
 #pragma unroll
-                for (int c = 0; c < D; ++c) d.y_out[c * n + i] = L.y[c];
+                for (int c = 0; c < D; ++c) gptr(d.y_out)[c * n + i] = L.y[c];
             }
-            if (d.status) d.status[i] = st;
+            if (d.status) gptr(d.status)[i] = st;
             if (d.counts) {
                 bg_counts cc;
                 cc.n_accepted = nacc;
                 cc.n_rejected = nrej;
-                d.counts[i] = cc;
+                gptr(d.counts)[i] = cc;
             }
             inst = -1;
         }
     }
+}
+
+// Residuals and loss from the states the forward pass stored at the
+// observations it reached (ws.res holds y(t_k) there): res = y - target in
+// place, loss = sum over k, c of res^2 in the forward's order, obs_y = y.
+// Kept out of the step loop: a target load at one lane's stop would stall
+// its whole wave, and in a wave some lane is at a stop on most attempts.
+template <int D>
+__global__ void adj_residual_kernel(const bg_adjoint_desc *__restrict__ ad, const AdjWs ws) {
+    const uint64_t n = ad->base.n;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t n_obs = ad->n_obs, reached = ws.nobs[i];
+    double loss = 0.0;
+    for (uint32_t k = 0; k < reached; ++k) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            const uint64_t o = ((uint64_t)k * D + c) * n + i;
+            double *r = ws.res + (i * n_obs + k) * D + c;
+            const double y = *r, rr = y - gptr(ad->targets)[o];
+            *r = rr;
+            loss = fma(rr, rr, loss);
+            if (ad->obs_y) gptr(ad->obs_y)[o] = y;
+        }
+    }
+    if (ad->loss) gptr(ad->loss)[i] = loss;
 }
 
 // Reverse of one accepted step of the explicit map
@@ -353,11 +355,11 @@ __global__ void __launch_bounds__(256)
                     inst = (int64_t)ws.order_b[k];
                     const uint64_t i = (uint64_t)inst;
 #pragma unroll
-                    for (int c = 0; c < RHS::P; ++c) L.p[c] = d.params[c * n + i];
-                    L.end = d.t_end ? d.t_end[i] : d.t_end_s;
-                    L.tol = d.tol ? d.tol[i] : d.tol_s;
-                    L.dt_min = d.dt_min ? d.dt_min[i] : d.dt_min_s;
-                    L.dt_max = d.dt_max ? d.dt_max[i] : d.dt_max_s;
+                    for (int c = 0; c < RHS::P; ++c) L.p[c] = gptr(d.params)[c * n + i];
+                    L.end = d.t_end ? gptr(d.t_end)[i] : d.t_end_s;
+                    L.tol = d.tol ? gptr(d.tol)[i] : d.tol_s;
+                    L.dt_min = d.dt_min ? gptr(d.dt_min)[i] : d.dt_min_s;
+                    L.dt_max = d.dt_max ? gptr(d.dt_max)[i] : d.dt_max_s;
 #pragma unroll
                     for (int c = 0; c < P; ++c) pb[c] = 0.0;
                     n_stops = count_stops(obs, n_obs, L.end);
@@ -365,7 +367,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
                     for (int c = 0; c < D; ++c)
                         lam[c] = final_obs ? 2.0 * ws.res[(i * n_obs + n_stops) * D + c] : 0.0;
-                    const int fw = d.status ? d.status[i] : BG_OK;
+                    const int fw = d.status ? gptr(d.status)[i] : BG_OK;
                     seg = fw == BG_OK ? (int)ws.nck[i] - 1 : -1;
                     load_res((int)n_stops - 1, i);
                     if (seg >= 0) load_ck(seg, i);
@@ -465,16 +467,16 @@ __global__ void __launch_bounds__(256)
         }
         if (inst >= 0 && fin) {
             const uint64_t i = (uint64_t)inst;
-            const int fw = d.status ? d.status[i] : BG_OK;
+            const int fw = d.status ? gptr(d.status)[i] : BG_OK;
             const double nan = __builtin_nan("");
             if (ad->grad_y0) {
 #pragma unroll
-                for (int c = 0; c < D; ++c) ad->grad_y0[c * n + i] = fw == BG_OK ? lam[c] : nan;
+                for (int c = 0; c < D; ++c) gptr(ad->grad_y0)[c * n + i] = fw == BG_OK ? lam[c] : nan;
             }
             if (ad->grad_params) {
 #pragma unroll
                 for (int c = 0; c < RHS::P; ++c)
-                    ad->grad_params[c * n + i] = fw == BG_OK ? pb[c] : nan;
+                    gptr(ad->grad_params)[c * n + i] = fw == BG_OK ? pb[c] : nan;
             }
             inst = -1;
         }
@@ -556,7 +558,7 @@ int rk_launch_adjoint(int method, int rhs, const bg_adjoint_desc &ad, hipStream_
             (const int32_t *)nullptr, (int32_t *)nullptr, (uint32_t)n, 0, 32, stream));
         const size_t bytes = 256 + sizeof(bg_adjoint_desc) + 64 +
                              sizeof(double) * (maxck * F * n + n_res + K * FT * slots) +
-                             sizeof(uint32_t) * n + 256 + 4 * (sizeof(uint32_t) * n + 256) +
+                             sizeof(uint32_t) * n + 256 + 5 * (sizeof(uint32_t) * n + 256) +
                              sort_bytes;
         char *buf = nullptr;
         BG_HIP_CHECK(hipMallocAsync((void **)&buf, bytes, stream));
@@ -580,10 +582,13 @@ int rk_launch_adjoint(int method, int rhs, const bg_adjoint_desc &ad, hipStream_
         q = align(q + sizeof(uint32_t) * n);
         int32_t *order_b = (int32_t *)q;
         q = align(q + sizeof(uint32_t) * n);
+        ws.nobs = (uint32_t *)q;
+        q = align(q + sizeof(uint32_t) * n);
         void *sort_tmp = q;
         ws.order_b = order_b;
         adj_setup_kernel<<<1, 64, 0, stream>>>(ws.counters, dd, ad);
         fwd<<<(uint32_t)grid_f, 256, 0, stream>>>(dd, ws);
+        adj_residual_kernel<D><<<(uint32_t)((n + 255) / 256), 256, 0, stream>>>(dd, ws);
         adj_order_keys_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, stream>>>(ws.nck, n, keys, idx);
         BG_HIP_CHECK(rocprim::radix_sort_pairs_desc(sort_tmp, sort_bytes, keys, keys_sorted, idx,
                                                     order_b, (uint32_t)n, 0, 32, stream));

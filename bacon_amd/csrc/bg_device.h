@@ -36,6 +36,20 @@ __device__ __forceinline__ double kc(double v) {
 }
 
 // ---------------------------------------------------------------------------
+// Caller buffers reached through a descriptor in device memory are generic
+// pointers, so the compiler emits FLAT accesses for them.  A FLAT access
+// counts on the LDS counter too: the next LDS read (the root8 table, staged
+// observation times) then waits until that global store or load completes.
+// gptr() states that the buffer is global memory, so the access is a
+// GLOBAL one and LDS waits no longer cover it.  Descriptor buffers are
+// always device (global) memory.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gptr(T *p) {
+    return (__attribute__((address_space(1))) T *)p;
+}
+
+// ---------------------------------------------------------------------------
 // Tableaux.  L is the reference's literal "Row i" data (rk.rs:459-502,
 // 580-601).  The reference stores it with Matrix::from_vec, which fills
 // column-major, and iterates row_iter, so its stage loop reads L transposed;
