@@ -317,9 +317,11 @@ typedef struct {
     uint32_t n_obs;
     uint32_t checkpoint_every;  /* K (0 = 16)                                   */
     uint32_t max_checkpoints;   /* per instance (0 = 128)                       */
-    uint32_t blocks_per_cu;     /* resident 256-thread blocks per CU, both
-                                   passes (0 = 2); base.order, if set, is the
-                                   forward pass's schedule                      */
+    uint32_t blocks_per_cu;     /* upper bound on resident 256-thread blocks
+                                   per CU for both passes (0: forward 2,
+                                   backward 1; the backward's LDS tape allows
+                                   one block per CU at K = 16); base.order,
+                                   if set, is the forward pass's schedule       */
     const double *obs_t;        /* [n_obs]                                      */
     const double *targets;      /* [n_obs][dim][n]                              */
     double *loss;               /* [n]                                          */
