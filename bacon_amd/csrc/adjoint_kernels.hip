@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(256)
                 bg_counts cc;
                 cc.n_accepted = nacc;
                 cc.n_rejected = nrej;
-                gptr(d.counts)[i] = cc;
+                d.counts[i] = cc;
             }
             inst = -1;
         }

@@ -82,6 +82,22 @@ struct Grp {
             return v;
         }
     }
+    // all-reduce maximum of unsigned keys (W == 32: one DPP max per stage)
+    __device__ __forceinline__ uint32_t max_u32(uint32_t v) const {
+        if constexpr (W == 32) {
+            v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false));
+            v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false));
+            v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false));
+            v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, false));
+            v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F));
+            return v;
+        } else {
+#pragma unroll
+            for (int m = W / 2; m >= 1; m >>= 1)
+                v = __builtin_elementwise_max(v, (uint32_t)__shfl_xor((int)v, m, W));
+            return v;
+        }
+    }
     // all-reduce maximum (IEEE maxNum: a NaN operand loses)
     __device__ __forceinline__ double max(double v) const {
         if constexpr (W == 32) {
@@ -115,6 +131,20 @@ __device__ __forceinline__ int pivot_lane(double key) {
 __device__ __forceinline__ bool pivot_nonzero(double key) {
     return (__builtin_bit_cast(uint64_t, key) & ~(uint64_t)31) != 0;
 }
+
+// The same choice on 32-bit keys: the high word of |a| (sign, exponent and
+// 20 mantissa bits) with its 5 lowest bits replaced by 31 - g, reduced by
+// one DPP integer max per stage.  The largest |a| wins up to 2^-15
+// relative (candidates that close are equally good pivots); ties go to the
+// lowest lane.  |a| below 2^-1017 counts as zero (SingularMatrix), as do
+// rows already used and NaN.
+__device__ __forceinline__ uint32_t pivot_key32(double a, bool eligible, int g) {
+    if (!eligible || !(a == a)) return 0u;
+    const uint32_t hi = (uint32_t)(__builtin_bit_cast(uint64_t, a) >> 32) & 0x7fffffffu;
+    return (hi & ~31u) | (uint32_t)(31 - g);
+}
+__device__ __forceinline__ int pivot_lane32(uint32_t key) { return 31 - (int)(key & 31u); }
+__device__ __forceinline__ bool pivot_nonzero32(uint32_t key) { return (key & ~31u) != 0u; }
 
 // ---------------------------------------------------------------------------
 // right-hand sides evaluated cooperatively: lane g returns f_g(t, y)
@@ -327,6 +357,13 @@ struct BdfLane {
 
     // SPLIT = 2: the same exchange steps on half rows.  Column k lives in
     // half k / WJ; the other half fetches a(g, k) across the halves.
+    // Normalisation is deferred: the pivot row is published unscaled with
+    // 1 / pivot beside it (LDS slot W, the padding column of row 0, unused
+    // until the final scatter), the other rows eliminate with
+    // f = a(g, k) / pivot, and each row is scaled by its own 1 / pivot once
+    // at the end.  A row that has been a pivot row holds pivot times its
+    // normalised values, so later updates stay consistent; this saves the
+    // per-step scaling of the pivot row.
     __device__ __forceinline__ bool invert_split() {
         LuT b[WJ];
 #pragma unroll
@@ -334,36 +371,38 @@ struct BdfLane {
         __builtin_amdgcn_wave_barrier();
         bool used = false, ok = true;
         int my_col = 0;
+        LuT my_inv = (LuT)1;
 #pragma unroll
         for (int k = 0; k < W; ++k) {
             const int hk = k / WJ, kk = k % WJ;
             const LuT mk = b[kk];
             const LuT other = (LuT)xor32_f64((double)mk);
             const LuT agk = (h == hk) ? mk : other;
-            const double key = grp.max(pivot_key((double)agk, !used, grp.g));
-            if (!pivot_nonzero(key)) ok = false;
-            const bool piv = (pivot_lane(key) == grp.g);
+            const uint32_t key = grp.max_u32(pivot_key32((double)agk, !used, grp.g));
+            if (!pivot_nonzero32(key)) ok = false;
+            const bool piv = (pivot_lane32(key) == grp.g);
             if (piv) {
                 used = true;
                 my_col = k;
-                const LuT ip = (LuT)1 / agk;
-#pragma unroll
-                for (int j = 0; j < WJ; ++j) b[j] *= ip;
-                if (h == hk) b[kk] = ip;
+                my_inv = (LuT)1 / agk;
 #pragma unroll
                 for (int j = 0; j < WJ; ++j) lds[h * WJ + j] = (double)b[j];
+                if (h == 0) lds[W] = (double)my_inv;
+                // column k of the inverse: pivot * (1 / pivot)
+                if (h == hk) b[kk] = (LuT)1;
             }
             __builtin_amdgcn_wave_barrier();
             if (!piv) {
-                const LuT f = agk;
+                const LuT f = agk * (LuT)lds[W];
 #pragma unroll
                 for (int j = 0; j < WJ; ++j) b[j] = fma(-f, (LuT)lds[h * WJ + j], b[j]);
-                // column k itself becomes -f / pivot (the published row holds
-                // 1 / pivot there)
-                if (h == hk) b[kk] = -f * (LuT)lds[k];
+                // column k of the inverse: -a(g, k) / pivot
+                if (h == hk) b[kk] = -f;
             }
             __builtin_amdgcn_wave_barrier();
         }
+#pragma unroll
+        for (int j = 0; j < WJ; ++j) b[j] *= my_inv;
 #pragma unroll
         for (int c = 0; c < WJ; ++c) lds[my_col * (W + 1) + h * WJ + c] = (double)b[c];
         lds[grp.g * (W + 1) + W] = (double)my_col;
