@@ -31,6 +31,10 @@ static constexpr uint32_t kBdfDefaultMaxSteps = 1000000u;
 //     i <- 15 - i within 16
 //   ds_swizzle bitmask 0x401F: i <- i ^ 16 within 32
 //   v_permlane32_swap(v, v): [0] = v[l mod 32], [1] = v[32 + l mod 32]
+//   v_permlane16_swap(v, v): [0] = v[32 (l / 32) + l mod 16],
+//     [1] = v[32 (l / 32) + 16 + l mod 16]
+// The permlane swaps are VALU operations; ds_swizzle goes through the LDS
+// unit and waits for it.
 // ---------------------------------------------------------------------------
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v) {
@@ -40,10 +44,14 @@ __device__ __forceinline__ double dpp_f64(double v) {
         (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), CTRL, 0xF, 0xF, false);
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
+// value of lane l ^ 16
+__device__ __forceinline__ uint32_t xor16_u32(uint32_t v) {
+    const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (__lane_id() & 16) ? p[0] : p[1];
+}
 __device__ __forceinline__ double xor16_f64(double v) {
     const uint64_t b = __builtin_bit_cast(uint64_t, v);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_swizzle((int)(uint32_t)b, 0x401F);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_swizzle((int)(uint32_t)(b >> 32), 0x401F);
+    const uint32_t lo = xor16_u32((uint32_t)b), hi = xor16_u32((uint32_t)(b >> 32));
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 // value of lane l ^ 32
@@ -89,7 +97,7 @@ struct Grp {
             v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false));
             v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false));
             v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, false));
-            v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F));
+            v = __builtin_elementwise_max(v, xor16_u32(v));
             return v;
         } else {
 #pragma unroll
