@@ -27,7 +27,7 @@ SENTINEL = -7
 
 def _t(a, dtype=None):
     import torch
-    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype or torch.float64, device=DEV)
+    return torch.as_tensor(np.array(a), dtype=dtype or torch.float64, device=DEV)
 
 
 def _bufs(w, n, t_end=None):
