@@ -8,6 +8,7 @@
 //   IVPIterator / collect_vec   src/ivp.rs:192-239
 //   RungeKutta45 / RungeKutta23 src/ivp/rk.rs:561, 656
 //   BDF6 / BDF2                 src/ivp/bdf.rs:706, 762
+//   Adams5 / Adams3             src/ivp/adams.rs:633, 693
 // A Rust setter returns Result<Self, IVPError>; here it returns *this or
 // throws IVPError, so builder chains read the same.  The derivative is a
 // registered device function id plus its parameters.
@@ -167,6 +168,8 @@ using RungeKutta23 = Solver<BG_RK23>;
 using RK45 = RungeKutta45;  // README.md:23-47 spelling
 using BDF6 = Solver<BG_BDF6>;
 using BDF2 = Solver<BG_BDF2>;
+using Adams5 = Solver<BG_ADAMS5>;
+using Adams3 = Solver<BG_ADAMS3>;
 
 }  // namespace ivp
 }  // namespace bacon

@@ -87,6 +87,26 @@ static void bdf_case(const char *name, int rhs, double y0, double t_end,
     for (auto &p : path) EXPECT(approx_eq(p.second[0], exact(p.first), 0.01), name);
 }
 
+template <class S>
+static void adams_case(const char *name, int rhs, double y0, double t_end, double dt_min,
+                       double dt_max, double tol, std::function<double(double)> exact,
+                       int semantics) {
+    // adams.rs:713-922
+    auto path = S::make(1)
+                    .with_minimum_dt(dt_min)
+                    .with_maximum_dt(dt_max)
+                    .with_tolerance(tol)
+                    .with_initial_time(0.0)
+                    .with_ending_time(t_end)
+                    .with_initial_conditions_slice({y0})
+                    .with_derivative(rhs)
+                    .with_semantics(semantics)
+                    .solve()
+                    .collect_vec();
+    EXPECT(!path.empty(), name);
+    for (auto &p : path) EXPECT(approx_eq(p.second[0], exact(p.first), 0.01), name);
+}
+
 static void builder_errors() {
     try {
         RK45::make(1).with_tolerance(0.0);
@@ -149,6 +169,16 @@ int main() {
         bdf_case<BDF6>("bdf6 sin", BG_RHS_COS, 0.0, 6.0, [](double t) { return std::sin(t); }, sem, empty);
         bdf_case<BDF2>("bdf2 exp", BG_RHS_EXP, 1.0, 7.0, [](double t) { return std::exp(t); }, sem, empty);
         bdf_case<BDF2>("bdf2 quadratic", BG_RHS_QUADRATIC, 1.0, 1.0, [](double t) { return 1.0 - t * t; }, sem, empty);
+        const double tau = 6.283185307179586;
+        auto ex = [](double t) { return std::exp(t); };
+        auto quad = [](double t) { return 1.0 - t * t; };
+        auto sine = [](double t) { return std::sin(t); };
+        adams_case<Adams5>("adams5 exp", BG_RHS_EXP, 1.0, 2.0, 1e-5, 0.1, 0.0005, ex, sem);
+        adams_case<Adams5>("adams5 quadratic", BG_RHS_QUADRATIC, 1.0, 5.0, 1e-7, 0.001, 0.01, quad, sem);
+        adams_case<Adams5>("adams5 sine", BG_RHS_COS, 0.0, tau, 1e-5, 0.001, 0.01, sine, sem);
+        adams_case<Adams3>("adams3 exp", BG_RHS_EXP, 1.0, 2.0, 1e-5, 0.1, 0.001, ex, sem);
+        adams_case<Adams3>("adams3 quadratic", BG_RHS_QUADRATIC, 1.0, 5.0, 1e-5, 0.001, 0.1, quad, sem);
+        adams_case<Adams3>("adams3 sine", BG_RHS_COS, 0.0, tau, 1e-5, 0.001, 0.01, sine, sem);
     }
     builder_errors();
     std::printf("%s (%d failures)\n", failures ? "FAILED" : "OK", failures);
