@@ -54,6 +54,7 @@ class EnsembleDesc(C.Structure):
 
 
 PATH_STOPS_ONLY = 1
+PATH_INSTANCE_MAJOR = 2
 
 
 class PathDesc(C.Structure):

@@ -870,14 +870,9 @@ static int visit_bdf(int method, int rhs, int semantics, uint32_t dim, int lu_fp
         return dim == 1 ? with_order(BTag<ScalarBdf<RhsFailAfter>>{}, BTag<double>{})
                         : BG_ERR_UNSUPPORTED;
     case BG_RHS_BRUSSELATOR: {
-        if (dim != 32 || method != BG_BDF6) return BG_ERR_UNSUPPORTED;
-        auto o7 = std::integral_constant<int, 7>{};
+        if (dim != 32) return BG_ERR_UNSUPPORTED;
         auto rt = BTag<RhsBrusselatorG<16>>{};
-        if (lu_fp32)
-            return semantics == BG_AS_BUILT ? v(rt, o7, BTag<BdfSem<true>>{}, BTag<float>{})
-                                            : v(rt, o7, BTag<BdfSem<false>>{}, BTag<float>{});
-        return semantics == BG_AS_BUILT ? v(rt, o7, BTag<BdfSem<true>>{}, BTag<double>{})
-                                        : v(rt, o7, BTag<BdfSem<false>>{}, BTag<double>{});
+        return lu_fp32 ? with_order(rt, BTag<float>{}) : with_order(rt, BTag<double>{});
     }
     default:
         return BG_ERR_UNSUPPORTED;

@@ -171,9 +171,19 @@ __global__ void __launch_bounds__(256)
     const uint32_t every = pd->every ? pd->every : 1u;
     const uint32_t capacity = pd->capacity;
     const bool stops_only = (pd->flags & BG_PATH_STOPS_ONLY) != 0;
-    const double *stops = pd->stops;
+    const bool inst_major = (pd->flags & BG_PATH_INSTANCE_MAJOR) != 0;
+    // global-memory views: FLAT stores would make the next LDS read (the
+    // root8 table) wait until they complete
+    const __attribute__((address_space(1))) double *stops = gptr(pd->stops);
+    __attribute__((address_space(1))) double *path_t = gptr(pd->path_t);
+    __attribute__((address_space(1))) double *path_y = gptr(pd->path_y);
+    __attribute__((address_space(1))) uint32_t *path_len = gptr(pd->path_len);
+    const uint32_t n_stops_all = pd->n_stops;
     int64_t inst = kNeedWork;
     uint32_t attempts = 0, nacc = 0, nrej = 0, next_stop = 0, n_stops = 0, nrec = 0;
+    uint32_t every_left = every;  // accepted steps until the next record
+    uint32_t slot = 0;            // ring entry of the next record (nrec % capacity)
+    double target = 0.0;          // the next stop, else the end
     for (;;) {
         const uint64_t need = __ballot(inst == kNeedWork);
         if (need != 0) {
@@ -190,9 +200,12 @@ __global__ void __launch_bounds__(256)
                     inst = order ? (int64_t)order[pos] : (int64_t)pos;
                     rk_load(L, desc_inputs(dv), (uint64_t)inst);
                     attempts = nacc = nrej = next_stop = nrec = 0;
+                    every_left = every;
+                    slot = 0;
                     n_stops = 0;
-                    while (n_stops < pd->n_stops && stops[n_stops] < L.end) ++n_stops;
+                    while (n_stops < n_stops_all && stops[n_stops] < L.end) ++n_stops;
                     while (next_stop < n_stops && stops[next_stop] <= L.t) ++next_stop;
+                    target = next_stop < n_stops ? stops[next_stop] : L.end;
                 } else {
                     inst = kExhausted;
                 }
@@ -207,7 +220,6 @@ __global__ void __launch_bounds__(256)
                     st = BG_MAXIMUM_ITERATIONS_EXCEEDED;
                 } else {
                     const bool to_stop = next_stop < n_stops;
-                    const double target = to_stop ? stops[next_stop] : L.end;
                     bool clamped;
                     const int r = L.attempt_to(target, clamped);
                     ++attempts;
@@ -217,17 +229,28 @@ __global__ void __launch_bounds__(256)
                         if (clamped && to_stop) {
                             L.t = target;
                             ++next_stop;
+                            target = next_stop < n_stops ? stops[next_stop] : L.end;
                             at_stop = true;
                         }
                         const bool last = L.t >= L.end;
-                        const bool rec = stops_only ? (at_stop || last) : (nacc % every == 0);
+                        --every_left;
+                        const bool rec = stops_only ? (at_stop || last) : (every_left == 0);
+                        if (every_left == 0) every_left = every;
                         if (rec) {
                             const uint64_t n = dv->n, i = (uint64_t)inst;
-                            const uint64_t e = capacity ? (uint64_t)(nrec % capacity) : 0;
                             if (capacity) {
-                                pd->path_t[e * n + i] = L.t;
+                                const uint64_t e = slot;
+                                if (inst_major) {
+                                    const uint64_t r = i * capacity + e;
+                                    path_t[r] = L.t;
 #pragma unroll
-                                for (int c = 0; c < D; ++c) pd->path_y[(e * D + c) * n + i] = L.y[c];
+                                    for (int c = 0; c < D; ++c) path_y[r * D + c] = L.y[c];
+                                } else {
+                                    path_t[e * n + i] = L.t;
+#pragma unroll
+                                    for (int c = 0; c < D; ++c) path_y[(e * D + c) * n + i] = L.y[c];
+                                }
+                                slot = slot + 1 == capacity ? 0u : slot + 1;
                             }
                             ++nrec;
                         }
@@ -242,7 +265,7 @@ __global__ void __launch_bounds__(256)
         }
         if (inst >= 0 && st >= 0) {
             rk_store(L, desc_outputs(dv), (uint64_t)inst, st, nacc, nrej);
-            if (pd->path_len) pd->path_len[inst] = nrec;
+            if (path_len) path_len[inst] = nrec;
             inst = kNeedWork;
         }
     }

@@ -339,11 +339,14 @@ def solve_ensemble(method, w: dict, semantics=L.INTENDED, plain=False, max_attem
 
 
 def solve_path(method, w: dict, capacity: int, every: int = 1, stops=None,
-               stops_only: bool = False, semantics=L.INTENDED, device: str = "cuda:0"):
+               stops_only: bool = False, semantics=L.INTENDED, device: str = "cuda:0",
+               instance_major: bool = False):
     """Ensemble solve with path output (bg_rk_solve_path_device) for a
     workload dict (see bacon_amd.workloads).  Returns numpy arrays: final t,
     y, status, counts, and the per-instance ring buffers path_t[capacity][n],
-    path_y[capacity][dim][n], path_len[n]."""
+    path_y[capacity][dim][n], path_len[n].  instance_major selects the
+    [n][capacity] device layout (BG_PATH_INSTANCE_MAJOR); the arrays are
+    returned in the same [capacity][...][n] shape either way."""
     import torch
     n, dim = w["y0"].shape[1], w["dim"]
 
@@ -360,8 +363,10 @@ def solve_path(method, w: dict, capacity: int, every: int = 1, stops=None,
                 y=torch.zeros((dim, n), dtype=torch.float64, device=device),
                 st=torch.full((n,), -1, dtype=torch.int32, device=device),
                 cnt=torch.zeros((n, 2), dtype=torch.int32, device=device),
-                pt=torch.zeros((max(capacity, 1), n), dtype=torch.float64, device=device),
-                py=torch.zeros((max(capacity, 1), dim, n), dtype=torch.float64, device=device),
+                pt=torch.zeros((n, max(capacity, 1)) if instance_major else (max(capacity, 1), n),
+                               dtype=torch.float64, device=device),
+                py=torch.zeros((n, max(capacity, 1), dim) if instance_major
+                               else (max(capacity, 1), dim, n), dtype=torch.float64, device=device),
                 pl=torch.zeros(n, dtype=torch.int32, device=device))
     st_arr = None
     if stops is not None and len(stops):
@@ -376,7 +381,8 @@ def solve_path(method, w: dict, capacity: int, every: int = 1, stops=None,
         counts=keep["cnt"].data_ptr())
     p = L.PathDesc(capacity=capacity, every=every,
                    n_stops=0 if st_arr is None else st_arr.numel(),
-                   flags=L.PATH_STOPS_ONLY if stops_only else 0,
+                   flags=(L.PATH_STOPS_ONLY if stops_only else 0)
+                   | (L.PATH_INSTANCE_MAJOR if instance_major else 0),
                    stops=None if st_arr is None else st_arr.data_ptr(),
                    path_t=keep["pt"].data_ptr(), path_y=keep["py"].data_ptr(),
                    path_len=keep["pl"].data_ptr())
@@ -384,7 +390,10 @@ def solve_path(method, w: dict, capacity: int, every: int = 1, stops=None,
                                            C.byref(p), None))
     torch.cuda.synchronize()
     cnt = keep["cnt"].cpu().numpy().astype(np.int64)
+    pt, py = keep["pt"].cpu().numpy(), keep["py"].cpu().numpy()
+    if instance_major:
+        pt, py = pt.T, py.transpose(1, 2, 0)
     return dict(t=keep["t"].cpu().numpy(), y=keep["y"].cpu().numpy(),
                 status=keep["st"].cpu().numpy(), n_accepted=cnt[:, 0], n_rejected=cnt[:, 1],
-                path_t=keep["pt"].cpu().numpy()[:capacity], path_y=keep["py"].cpu().numpy()[:capacity],
+                path_t=pt[:capacity], path_y=py[:capacity],
                 path_len=keep["pl"].cpu().numpy().astype(np.int64))

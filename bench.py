@@ -17,6 +17,8 @@ The JSON line also carries:
     FP64 and in FP32;
   * batched RK45 on 65,536 linear systems (N = 128) sharing one matrix, the
     stage right-hand sides as FP64 MFMA GEMMs;
+  * the other methods and outputs on one shared Lorenz workload: RK23,
+    Adams5, path output, the solve_ivp chain, and BDF2;
   * the CPU oracle timed on the host cores over a bounded sample.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1]
@@ -42,6 +44,8 @@ PEAK_HBM_GBS = 8000.0
 # 2D(nnz(A)+nnz(e)+nnz(b)) + 6(F_rhs + D) + (2D + 2) + F_ctrl
 FLOPS_ATTEMPT_LORENZ = 2 * 3 * (15 + 5 + 4) + 6 * (8 + 3) + 8 + 6   # 224
 FLOPS_ATTEMPT_LV = 2 * 2 * (15 + 5 + 4) + 6 * (7 + 2) + 6 + 6       # 162
+# Bogacki-Shampine 2(3): nnz(A) = 5, nnz(e) = 4, nnz(b) = 3, 4 stages
+FLOPS_ATTEMPT_LORENZ_RK23 = 2 * 3 * (5 + 4 + 3) + 4 * (8 + 3) + 8 + 6  # 130
 # reverse of one accepted LV step: stage recompute (6 RHS, 15 nnz(A) combos,
 # 6 dt scalings) + 6 VJPs + stage-adjoint combos + b-weighting + sums
 FLOPS_VJP_STEP_LV = (6 * 7 + 2 * 2 * 15 + 6 * 2) + 6 * 20 + 2 * 2 * 15 + 2 * 2 * 4 + 6 * 2 * 2
@@ -157,6 +161,133 @@ def plain_path(cx, w, m):
     return m / t, b
 
 
+def method_family(cx, args):
+    """The methods and outputs beyond the headline on one shared workload:
+    262,144 plain Lorenz-63 instances (t in [0, 10], tol 1e-6, dt in
+    [1e-6, 0.1]).  RK45 as the reference point; RK23; Adams5; RK45 with every
+    accepted step recorded in a 64-entry ring per instance; the solve_ivp
+    chain (Adams5, then RK45, then BDF6) with a 5000-step cap that sends the
+    longer Adams5 instances on to RK45; BDF2 on 16,384 Brusselator instances
+    to t = 2 at tol 1e-5 (second order needs ~15,000 steps there at 1e-7).
+    RK23 runs to t = 1: a third-order pair at tol 1e-6 takes ~10^5 attempts
+    per unit time on this system.  Inputs resident in HBM except for solve_ivp, which runs
+    through the host-array builder."""
+    from bacon_amd import ivp
+    from bacon_amd import workloads as W
+    torch, L = cx.torch, cx.L
+    nf = 262144
+    log(f"[bench] method family, {nf} plain Lorenz instances")
+    wf = W.lorenz_plain(nf)
+    bf, df = rk_ensemble(cx, wf, order=False)
+    out = {"workload": "262,144 Lorenz-63 instances, sigma 10, beta 8/3, rho~U[20,40], "
+                       "t in [0,10], tol 1e-6, dt_min 1e-6, dt_max 0.1"}
+
+    def entry(wall, kern, flops_per_attempt=None):
+        c = bf["counts"].cpu().numpy().astype(np.int64)
+        e = {"solves_per_s": nf / wall, "ms_per_solve": wall * 1e3,
+             "failed_instances": int(np.count_nonzero(bf["status"].cpu().numpy())),
+             "accepted_steps_per_s": int(c[:, 0].sum()) / wall,
+             "attempts": int(c.sum())}
+        if flops_per_attempt:
+            tf = int(c.sum()) * flops_per_attempt / kern / 1e12
+            e.update(fp64_tflops=tf, frac_of_fp64_peak=tf / PEAK_FP64_TFLOPS,
+                     flops_per_attempt=flops_per_attempt)
+        return e
+
+    for name, method, fpa, t_end in (("rk45", L.RK45, FLOPS_ATTEMPT_LORENZ, None),
+                                     ("rk23_to_t1", L.RK23, FLOPS_ATTEMPT_LORENZ_RK23, 1.0)):
+        dm = L.EnsembleDesc.from_buffer_copy(df)
+        if t_end is not None:
+            dm.t_end, dm.t_end_s = None, t_end
+
+        def run():
+            rc = cx.lib.bg_rk_solve_device(method, L.RHS["lorenz"], L.INTENDED, C.byref(dm),
+                                           None, cx.sh)
+            if rc != 0:
+                raise RuntimeError(rc)
+        wall, kern = time_launches(cx, run, 2, 1)
+        out[name] = entry(wall, kern, fpa)
+
+    def run_adams():
+        rc = cx.lib.bg_adams_solve_device(L.ADAMS5, L.RHS["lorenz"], L.INTENDED, C.byref(df),
+                                          cx.sh)
+        if rc != 0:
+            raise RuntimeError(rc)
+    wall, kern = time_launches(cx, run_adams, 2, 1)
+    out["adams5"] = entry(wall, kern)
+    out["adams5"]["note"] = "accepted steps count every yield, bootstrap replays included"
+
+    cap = 64
+    pb = dict(t=cx.zeros((cap * nf,)), y=cx.zeros((cap * 3 * nf,)),
+              len=cx.zeros(nf, torch.int32))
+    for key, flags in (("rk45_path_every_step", 0),
+                       ("rk45_path_every_step_instance_major", L.PATH_INSTANCE_MAJOR)):
+        pd = L.PathDesc(capacity=cap, every=1, flags=flags, path_t=pb["t"].data_ptr(),
+                        path_y=pb["y"].data_ptr(), path_len=pb["len"].data_ptr())
+
+        def run_path():
+            rc = cx.lib.bg_rk_solve_path_device(L.RK45, L.RHS["lorenz"], L.INTENDED,
+                                                C.byref(df), C.byref(pd), cx.sh)
+            if rc != 0:
+                raise RuntimeError(rc)
+        wall, kern = time_launches(cx, run_path, 2, 1)
+        e = entry(wall, kern, FLOPS_ATTEMPT_LORENZ)
+        recorded = int(pb["len"].cpu().numpy().astype(np.int64).sum())
+        e.update(path_entries_per_s=recorded / wall,
+                 path_write_gb_per_s=recorded * 32 / kern / 1e9,
+                 overhead_vs_rk45=wall / (out["rk45"]["ms_per_solve"] / 1e3) - 1.0,
+                 ring_capacity=cap)
+        out[key] = e
+    del pb
+
+    m = 65536
+    sub = {k: (v[..., :m] if isinstance(v, np.ndarray) and v.ndim and v.shape[-1] == nf else v)
+           for k, v in wf.items()}
+    ens = (ivp.Ensemble(L.ADAMS5, 3, m).with_minimum_dt(sub["dt_min"])
+           .with_maximum_dt(sub["dt_max"]).with_tolerance(sub["tol"])
+           .with_initial_time(0.0).with_ending_time(sub["t_end"])
+           .with_initial_conditions(sub["y0"]).with_derivative("lorenz", sub["params"])
+           .with_max_attempts(5000))
+    ens.solve_ivp()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    r = ens.solve_ivp()
+    t = time.perf_counter() - t
+    used = r["method_used"]
+    out["solve_ivp_chain"] = {
+        "instances": m, "solves_per_s": m / t, "ms_per_solve": t * 1e3,
+        "step_cap": 5000, "host_arrays": True,
+        "finished_by": {"adams5": int((used == L.ADAMS5).sum()),
+                        "rk45": int((used == L.RK45).sum()),
+                        "bdf6": int((used == L.BDF6).sum())},
+        "failed_instances": int(np.count_nonzero(r["status"]))}
+    del ens
+
+    nb2 = 16384
+    wb = W.brusselator(nb2)
+    bb = dict(y0=cx.T(wb["y0"]), params=cx.T(wb["params"]), t_out=cx.zeros(nb2),
+              y_out=cx.zeros((32, nb2)), status=cx.zeros(nb2, torch.int32),
+              counts=cx.zeros((nb2, 2), torch.int32))
+    db = L.EnsembleDesc(
+        n=nb2, dim=32, n_params=3, y0=bb["y0"].data_ptr(), params=bb["params"].data_ptr(),
+        t0_s=0.0, t_end_s=2.0, tol_s=1e-5, dt_min_s=1e-12, dt_max_s=0.1,
+        t_out=bb["t_out"].data_ptr(), y_out=bb["y_out"].data_ptr(),
+        status=bb["status"].data_ptr(), counts=bb["counts"].data_ptr())
+
+    def run_b2():
+        rc = cx.lib.bg_bdf_solve_device(L.BDF2, L.RHS["brusselator"], L.INTENDED, 64,
+                                        C.byref(db), cx.sh)
+        if rc != 0:
+            raise RuntimeError(rc)
+    wall, kern = time_launches(cx, run_b2, 1, 1)
+    out["bdf2_brusselator"] = {
+        "instances": nb2, "t_end": 2.0, "tol": 1e-5,
+        "solves_per_s": nb2 / wall, "ms_per_solve": wall * 1e3,
+        "failed_instances": int(np.count_nonzero(bb["status"].cpu().numpy())),
+        "accepted_steps": int(bb["counts"].cpu().numpy()[:, 0].sum())}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,8 +299,8 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--extras", default="tol8,adjoint,bdf,linear",
-                    help="comma list of extra measurements: tol8, adjoint, bdf, linear")
+    ap.add_argument("--extras", default="tol8,adjoint,bdf,linear,family",
+                    help="comma list of extra measurements: tol8, adjoint, bdf, linear, family")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline ensemble (used for profiling)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -463,6 +594,9 @@ def main():
                                  "sample": f"first {m} instances"}
         result["linear_shared_mfma"] = lin
         del bl, dl
+
+    if "family" in extras:
+        result["method_family"] = method_family(cx, args)
 
     if not args.no_cpu_baseline:
         import oracle as O

@@ -103,6 +103,21 @@ def test_brusselator_ensemble_vs_oracle(gpu, orc, bruss_oracle, lu_bits):
     assert np.all(g["n_lu"] > 0) and np.all(g["n_newton"] > 0)
 
 
+def test_brusselator_bdf2_vs_oracle(gpu, orc):
+    """BDF2 (bdf.rs:708-762) on the same kernel: 8 Brusselator instances to
+    t = 2, within 10 tol of the oracle, accepted counts within 5 %."""
+    w = _bruss(8)
+    w["t_end"] = np.full(8, 2.0)
+    o = O.solve_ensemble(O.BDF2, "brusselator", w["y0"], w["params"], w["t_end"], w["tol"],
+                         dt_min=w["dt_min"], dt_max=w["dt_max"], semantics=O.INTENDED)
+    g = ivp.solve_ensemble(L.BDF2, w)
+    assert np.all(o["status"] == 0) and np.all(g["status"] == 0)
+    assert np.all(np.abs(g["t"] - 2.0) < 1e-9)
+    err = np.abs(g["y"] - o["y"]).max()
+    assert err < 10 * 1e-7, err
+    assert np.abs(g["n_accepted"] - o["n_accepted"]).max() <= 0.05 * o["n_accepted"].max()
+
+
 def test_brusselator_vs_radau(gpu, orc):
     gold = np.load("tests/golden/bruss_golden.npz")
     w = _bruss(8)

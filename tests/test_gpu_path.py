@@ -7,7 +7,8 @@ compared bit for bit against the oracle's DEVICE profile:
   * every accepted step, on 96 Lorenz-63 instances of the plain workload;
   * with shared stop times (exact landings, the same path with stops added);
   * stop-only recording (dense output at given times);
-  * decimation (every k-th step) and ring wrap-around.
+  * decimation (every k-th step) and ring wrap-around;
+  * the instance-major layout holds the same entries.
 """
 import numpy as np
 import pytest
@@ -96,3 +97,12 @@ def test_invalid_arguments(gpu, w):
     p = L.PathDesc(capacity=8)  # no buffers
     assert L.lib().bg_rk_solve_path_device(L.RK45, L.RHS["lorenz"], L.INTENDED, C.byref(d),
                                            C.byref(p), None) == -1
+
+
+def test_instance_major_layout_same_entries(gpu, w):
+    for kw in (dict(capacity=4096), dict(capacity=16), dict(capacity=16, stops=STOPS,
+                                                           stops_only=True)):
+        a = ivp.solve_path(L.RK45, w, **kw)
+        b = ivp.solve_path(L.RK45, w, instance_major=True, **kw)
+        for k in ("path_t", "path_y", "path_len", "y", "t", "status"):
+            assert np.array_equal(a[k], b[k]), (kw, k)
