@@ -50,7 +50,7 @@ CPP_TEST_BIN = os.path.join(os.path.dirname(HERE), "tests", "cpp", "test_referen
 
 def build_cpp_tests(verbose: bool = True) -> str:
     """The C++ restatement of the reference's ivp tests, linked against the
-    library (runs on a GPU box from tests/test_gpu_cpp.py)."""
+    library (run on a GPU by tests/test_gpu_cpp.py)."""
     cmd = ["g++", "-std=c++17", "-O2", "-o", CPP_TEST_BIN, CPP_TEST_SRC,
            f"-L{HERE}", "-lbacon_gpu", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../bacon_amd"]
     r = subprocess.run(cmd, capture_output=True, text=True)

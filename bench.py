@@ -311,7 +311,8 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(instances are independent, see DESIGN.md for how the path shards)",
+              "(instances are independent: a multi-GPU run gives each GPU a contiguous "
+              "range of the longest-first order, with no exchange between GPUs)",
               file=sys.stderr)
         sys.exit(2)
 
