@@ -161,6 +161,38 @@ def plain_path(cx, w, m):
     return m / t, b
 
 
+def tight_error(cx, w, y_out, m=256, rhs="lorenz"):
+    """Final-state error of the first m instances against the same instances
+    solved at tol 1e-13 (FP64, this library's RK45, dt_min 1e-15).  y_out:
+    the run's [dim][n] final states on the device."""
+    L = cx.L
+    n = w["y0"].shape[1]
+    sub = {k: (v[..., :m] if isinstance(v, np.ndarray) and v.ndim and v.shape[-1] == n else v)
+           for k, v in w.items()}
+    dim = w["dim"]
+    bt = dict(y0=cx.T(sub["y0"]), params=cx.T(sub["params"]),
+              tend=cx.T(np.broadcast_to(np.asarray(sub["t_end"], dtype=np.float64), (m,))),
+              t=cx.zeros(m), y=cx.zeros((dim, m)), st=cx.zeros(m, cx.torch.int32),
+              cnt=cx.zeros((m, 2), cx.torch.int32))
+    dt = L.EnsembleDesc(
+        n=m, dim=dim, n_params=bt["params"].shape[0], y0=bt["y0"].data_ptr(),
+        params=bt["params"].data_ptr(), t0_s=0.0, t_end=bt["tend"].data_ptr(), tol_s=1e-13,
+        dt_min_s=1e-15, dt_max_s=w["dt_max"], t_out=bt["t"].data_ptr(),
+        y_out=bt["y"].data_ptr(), status=bt["st"].data_ptr(), counts=bt["cnt"].data_ptr())
+    rc = cx.lib.bg_rk_solve_device(L.RK45, L.RHS[rhs], L.INTENDED, C.byref(dt), None, cx.sh)
+    if rc != 0:
+        raise RuntimeError(f"tight solve returned {rc}")
+    yt = bt["y"].cpu().numpy()
+    yr = y_out[:, :m].cpu().numpy()
+    err = np.abs(yr - yt).max(axis=0)
+    rel = err / np.maximum(1.0, np.abs(yt).max(axis=0))
+    return {"reference": "the same instances at tol 1e-13 (FP64 RK45, this library)",
+            "sample": f"first {m} instances",
+            "tight_solve_failed": int(np.count_nonzero(bt["st"].cpu().numpy())),
+            "max_abs": float(err.max()), "median_abs": float(np.median(err)),
+            "max_rel": float(rel.max()), "median_rel": float(np.median(rel))}
+
+
 def method_family(cx, args):
     """The methods and outputs beyond the headline on one shared workload:
     262,144 plain Lorenz-63 instances (t in [0, 10], tol 1e-6, dt in
@@ -207,6 +239,8 @@ def method_family(cx, args):
                 raise RuntimeError(rc)
         wall, kern = time_launches(cx, run, 2, 1)
         out[name] = entry(wall, kern, fpa)
+        if method == L.RK45:
+            out[name]["error_vs_tol_1e-13"] = tight_error(cx, wf, bf["y_out"])
 
     def run_adams():
         rc = cx.lib.bg_adams_solve_device(L.ADAMS5, L.RHS["lorenz"], L.INTENDED, C.byref(df),
@@ -425,6 +459,11 @@ def main():
         },
     }
 
+    result["error_vs_tol_1e-13"] = tight_error(cx, w, b["y_out"])
+    result["error_vs_tol_1e-13"]["note"] = (
+        "Lorenz-63 is chaotic: a local error of the tolerance grows ~e^(0.9 t) by t_end "
+        "(up to 20), so late final states differ at O(1) from any other solver's")
+
     if not args.no_plain:
         m = 64
         log(f"[bench] plain per-instance path, {m} instances")
@@ -456,7 +495,8 @@ def main():
                 "fp64_tflops": int(cu.sum()) * FLOPS_ATTEMPT_LORENZ / kern_u / 1e12,
                 "frac_of_fp64_peak": int(cu.sum()) * FLOPS_ATTEMPT_LORENZ / kern_u / 1e12
                                      / PEAK_FP64_TFLOPS,
-                "tuning": tnu.as_dict()}
+                "tuning": tnu.as_dict(),
+                "error_vs_tol_1e-13": tight_error(cx, wu, bu["y_out"])}
         if not args.no_plain:
             plain_u, pbu = plain_path(cx, wu, 64)
             tol8["plain_gpu_solves_per_s"] = plain_u
@@ -499,6 +539,15 @@ def main():
             "accepted_steps": acc_a,
             "fp64_tflops": flops_a / kern_a / 1e12,
         }
+        gold = np.load(os.path.join(ROOT, "tests", "golden", "lv_golden.npz"))
+        mg = gold["y0"].shape[1]
+        if np.array_equal(gold["y0"], wl["y0"][:, :mg]) and np.array_equal(gold["params"],
+                                                                         wl["params"][:, :mg]):
+            oy = r["obs_y"][:, :, :mg].cpu().numpy()      # [n_obs][2][mg]
+            ey = np.abs(oy - gold["y"].transpose(1, 2, 0))
+            result["adjoint"]["obs_error_vs_dop853_1e-13"] = {
+                "sample": f"first {mg} instances x 20 observations",
+                "max_abs": float(ey.max()), "median_abs": float(np.median(ey))}
         del r, res
 
     if "bdf" in extras:
@@ -537,6 +586,13 @@ def main():
                      "tflops": flops_b / kern_b / 1e12}
             if bits == 64:
                 y64 = yb
+                gold = np.load(os.path.join(ROOT, "tests", "golden", "bruss_golden.npz"))
+                mg = gold["y0"].shape[1]
+                if np.array_equal(gold["y0"], wb["y0"][:, :mg]):
+                    eb = np.abs(yb[:, :mg] - gold["y"][:, 1, :].T)
+                    entry["error_vs_radau_1e-12"] = {
+                        "sample": f"first {mg} instances at t = 10",
+                        "max_abs": float(eb.max()), "median_abs": float(np.median(eb))}
             else:
                 entry["max_abs_diff_vs_fp64_lu"] = float(np.abs(yb - y64).max())
             bdf[f"lu_fp{bits}"] = entry
