@@ -551,8 +551,9 @@ int adams_launch_ensemble(int method, int rhs, int semantics, const bg_ensemble_
         bg_ensemble_desc *ddesc = (bg_ensemble_desc *)(ws + 256);
         adams_setup_kernel<<<1, 64, 0, stream>>>(counter, ddesc, d);
         adams_ensemble_kernel<R, O, AB><<<(uint32_t)grid, 256, 0, stream>>>(ddesc, counter);
-        BG_HIP_CHECK(hipGetLastError());
+        const hipError_t launched = hipGetLastError();
         if (!ws_in) BG_HIP_CHECK(hipFreeAsync(ws, stream));
+        BG_HIP_CHECK(launched);
         return BG_OK;
     });
 }

@@ -590,11 +590,15 @@ int rk_launch_adjoint(int method, int rhs, const bg_adjoint_desc &ad, hipStream_
         fwd<<<(uint32_t)grid_f, 256, 0, stream>>>(dd, ws);
         adj_residual_kernel<D><<<(uint32_t)((n + 255) / 256), 256, 0, stream>>>(dd, ws);
         adj_order_keys_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, stream>>>(ws.nck, n, keys, idx);
-        BG_HIP_CHECK(rocprim::radix_sort_pairs_desc(sort_tmp, sort_bytes, keys, keys_sorted, idx,
-                                                    order_b, (uint32_t)n, 0, 32, stream));
-        bwd<<<(uint32_t)grid_b, 256, dyn_lds, stream>>>(dd, ws);
-        BG_HIP_CHECK(hipGetLastError());
+        hipError_t e = rocprim::radix_sort_pairs_desc(sort_tmp, sort_bytes, keys, keys_sorted,
+                                                      idx, order_b, (uint32_t)n, 0, 32, stream);
+        if (e == hipSuccess) {
+            bwd<<<(uint32_t)grid_b, 256, dyn_lds, stream>>>(dd, ws);
+            e = hipGetLastError();
+        }
+        // the workspace goes back on every path
         BG_HIP_CHECK(hipFreeAsync(buf, stream));
+        BG_HIP_CHECK(e);
         return BG_OK;
     };
     switch (rhs) {

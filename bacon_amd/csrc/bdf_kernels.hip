@@ -918,8 +918,9 @@ int bdf_launch_ensemble(int method, int rhs, int semantics, int lu_fp32, const b
                          bdf_setup_kernel<<<1, 64, 0, stream>>>(counter, ddesc, d);
                          bdf_ensemble_kernel<R, O, S, F, SP>
                              <<<(uint32_t)grid, 64, 0, stream>>>(ddesc, counter);
-                         BG_HIP_CHECK(hipGetLastError());
+                         const hipError_t launched = hipGetLastError();
                          if (!ws_in) BG_HIP_CHECK(hipFreeAsync(ws, stream));
+                         BG_HIP_CHECK(launched);
                          return BG_OK;
                      });
 }

@@ -274,8 +274,9 @@ int linear_launch_ensemble(int method, int semantics, const bg_ensemble_desc &d,
         bg_ensemble_desc *ddesc = (bg_ensemble_desc *)(ws + 256);
         linear_setup_kernel<<<1, 64, 0, stream>>>(counter, ddesc, d);
         kern<<<(uint32_t)grid, 256, 0, stream>>>(ddesc, counter);
-        BG_HIP_CHECK(hipGetLastError());
+        const hipError_t launched = hipGetLastError();
         if (!ws_in) BG_HIP_CHECK(hipFreeAsync(ws, stream));
+        BG_HIP_CHECK(launched);
         return BG_OK;
     };
     if (semantics == BG_AS_BUILT) return run(linear_rk_kernel<Tableau<Fehlberg45, true>>);

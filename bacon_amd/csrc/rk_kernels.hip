@@ -442,8 +442,9 @@ int rk_launch_ensemble(int method, int rhs, int semantics, const bg_ensemble_des
         const uint32_t refill_min = lc.refill_min ? lc.refill_min : 1u;
         rk_ensemble_kernel<R, T>
             <<<(uint32_t)grid, block, 0, stream>>>(ddesc, counter, refill_min, lc.prio_cutoff);
-        BG_HIP_CHECK(hipGetLastError());
+        const hipError_t launched = hipGetLastError();
         if (!ws_in) BG_HIP_CHECK(hipFreeAsync(ws, stream));
+        BG_HIP_CHECK(launched);
         return BG_OK;
     });
 }
@@ -512,8 +513,9 @@ int rk_launch_path_ensemble(int method, int rhs, int semantics, const bg_ensembl
         bg_path_desc *pdesc = (bg_path_desc *)(ws + off_p);
         rk_path_setup_kernel<<<1, 64, 0, stream>>>(counter, ddesc, d, pdesc, p);
         rk_path_ensemble_kernel<R, T><<<(uint32_t)grid, 256, 0, stream>>>(ddesc, pdesc, counter);
-        BG_HIP_CHECK(hipGetLastError());
+        const hipError_t launched = hipGetLastError();
         BG_HIP_CHECK(hipFreeAsync(ws, stream));
+        BG_HIP_CHECK(launched);
         return BG_OK;
     });
 }

@@ -106,10 +106,13 @@ static int schedule(const bg_tuning &tn, const bg_ensemble_desc &d, int32_t *ord
     int32_t *idx = (int32_t *)(ws + 2 * kbytes);
     void *tmp = ws + 2 * kbytes + ibytes;
     tune_cost_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, stream>>>(d, model_of(tn), keys, idx);
-    BG_HIP_CHECK(hipGetLastError());
-    BG_HIP_CHECK(rocprim::radix_sort_pairs_desc(tmp, sort_bytes, keys, keys_sorted, idx, order,
-                                                (uint32_t)n, 0, 32, stream));
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = rocprim::radix_sort_pairs_desc(tmp, sort_bytes, keys, keys_sorted, idx, order,
+                                           (uint32_t)n, 0, 32, stream);
+    // the workspace goes back on every path
     BG_HIP_CHECK(hipFreeAsync(ws, stream));
+    BG_HIP_CHECK(e);
     return BG_OK;
 }
 
@@ -213,6 +216,13 @@ int rk_autotune(int method, int rhs, int semantics, const bg_ensemble_desc &din,
     if (n < 2 || rhs == BG_RHS_LINEAR) return BG_OK;  // defaults: nothing to schedule
     if (n > 0x7fffffffull || !din.y0 || (P && !din.params)) return BG_ERR_INVALID_ARGUMENT;
     if (din.order) return BG_ERR_INVALID_ARGUMENT;   // the tuner owns the schedule
+    // inside the do/while below: record the failure and fall through to the
+    // frees after the loop
+#define BG_TRY(expr)                  \
+    if ((expr) != hipSuccess) {       \
+        rc = BG_ERR_HIP;              \
+        break;                        \
+    }
 
     bg_ensemble_desc d = din;
     // step counts: the caller's buffer or a scratch one
@@ -236,17 +246,17 @@ int rk_autotune(int method, int rhs, int semantics, const bg_ensemble_desc &din,
         const uint32_t m = (uint32_t)(n < 16384 ? n : 16384);
         const uint64_t stride = n / m;
         const uint32_t W = P + 3;
-        BG_HIP_CHECK(hipMallocAsync((void **)&rows_d, (size_t)m * W * sizeof(double), stream));
+        BG_TRY(hipMallocAsync((void **)&rows_d, (size_t)m * W * sizeof(double), stream));
         tune_sample_kernel<<<(m + 255) / 256, 256, 0, stream>>>(d, d.counts, stride, m, P, rows_d);
-        BG_HIP_CHECK(hipGetLastError());
+        BG_TRY(hipGetLastError());
         std::vector<double> rows((size_t)m * W);
-        BG_HIP_CHECK(hipMemcpyAsync(rows.data(), rows_d, rows.size() * sizeof(double),
+        BG_TRY(hipMemcpyAsync(rows.data(), rows_d, rows.size() * sizeof(double),
                                     hipMemcpyDeviceToHost, stream));
-        BG_HIP_CHECK(hipStreamSynchronize(stream));
+        BG_TRY(hipStreamSynchronize(stream));
         tn.pilot_n = m;
         fit_model(rows, m, P, tn);
         // 3. schedule, then time the candidate shapes on the whole problem
-        BG_HIP_CHECK(hipMallocAsync((void **)&order, n * sizeof(int32_t), stream));
+        BG_TRY(hipMallocAsync((void **)&order, n * sizeof(int32_t), stream));
         if ((rc = schedule(tn, d, order, stream)) != BG_OK) break;
         d.order = order;
         const uint32_t cand[][2] = {{256, 0}, {256, 4}, {256, 3}, {256, 2}, {128, 6}, {64, 12}};
@@ -268,6 +278,7 @@ int rk_autotune(int method, int rhs, int semantics, const bg_ensemble_desc &din,
         if (rc != BG_OK) break;
         tn.best_ms = best;
     } while (0);
+#undef BG_TRY
     if (rows_d) hipFreeAsync(rows_d, stream);
     if (order) hipFreeAsync(order, stream);
     if (scratch_counts) hipFreeAsync(scratch_counts, stream);
