@@ -81,8 +81,7 @@ class Ctx:
 
     def T(self, a, dt=None):
         torch = self.torch
-        return torch.as_tensor(np.ascontiguousarray(a), dtype=dt or torch.float64,
-                               device=self.dev)
+        return torch.as_tensor(np.array(a), dtype=dt or torch.float64, device=self.dev)
 
     def zeros(self, shape, dt=None):
         return self.torch.zeros(shape, dtype=dt or self.torch.float64, device=self.dev)
