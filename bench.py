@@ -450,6 +450,11 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
             "traffic": MEASURED_TRAFFIC_BYTES,
+            "traffic_note": "rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (factors from a "
+                            "1 GiB copy kernel); above the algorithmic bytes because the "
+                            "longest-first schedule gathers each instance's inputs and scatters "
+                            "its outputs through order[k], one cache line per 8-byte value: "
+                            "~0.26 ms of HBM time in a compute-bound 59 ms kernel",
             "kernel": "rk_ensemble_kernel<RhsLorenz, Tableau<Fehlberg45,false>>",
             "flops_per_attempt": FLOPS_ATTEMPT_LORENZ,
             "avg_kernel_ms": kern * 1e3,
