@@ -143,6 +143,18 @@ def time_launches(cx, fn, steps, warmup):
     return wall, float(np.mean([s.elapsed_time(e) for s, e in ev])) / 1000.0
 
 
+def cpu_model():
+    """The host CPU's model name, from /proc/cpuinfo."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def plain_path(cx, w, m):
     """The plain per-instance path on the first m instances of workload w."""
     n = w["y0"].shape[1]
@@ -469,7 +481,7 @@ def main():
         "(up to 20), so late final states differ at O(1) from any other solver's")
 
     if not args.no_plain:
-        m = 64
+        m = 1024
         log(f"[bench] plain per-instance path, {m} instances")
         plain_sps, pb = plain_path(cx, w, m)
         same = bool(np.array_equal(pb["y_out"].cpu().numpy(),
@@ -502,7 +514,7 @@ def main():
                 "tuning": tnu.as_dict(),
                 "error_vs_tol_1e-13": tight_error(cx, wu, bu["y_out"])}
         if not args.no_plain:
-            plain_u, pbu = plain_path(cx, wu, 64)
+            plain_u, pbu = plain_path(cx, wu, 1024)
             tol8["plain_gpu_solves_per_s"] = plain_u
             tol8["speedup_vs_plain"] = (n / wall_u) / plain_u
             tol8["identical_to_batched"] = bool(np.array_equal(
@@ -672,12 +684,27 @@ def main():
                              profile=O.PROFILE_BACON, threads=threads)
             done += chunk
         tc = time.perf_counter() - tc
+        # bacon itself is single-threaded: the same oracle on one thread, on a
+        # shorter sample of the same instances
+        done1, chunk1 = 0, 64
+        t1 = time.perf_counter()
+        while time.perf_counter() - t1 < args.cpu_seconds / 4 and done1 < n:
+            s = W.lorenz_batched(chunk1, first=done1)
+            O.solve_ensemble(O.RK45, "lorenz", s["y0"], s["params"], s["t_end"], s["tol"],
+                             dt_min=w["dt_min"], dt_max=w["dt_max"], semantics=O.INTENDED,
+                             profile=O.PROFILE_BACON, threads=1)
+            done1 += chunk1
+        t1 = time.perf_counter() - t1
         result["cpu_baseline"] = {
             "value": done / tc, "unit": "solves/s", "cores": threads, "kind": "port",
             "sample": f"first {done} instances of the same workload, C oracle "
                       f"(bacon arithmetic, -O2), OpenMP over {threads} threads, {tc:.1f} s",
+            "single_thread": {"value": done1 / t1, "unit": "solves/s",
+                              "sample": f"first {done1} instances, one thread, {t1:.1f} s"},
+            "host": {"cpu_model": cpu_model(), "nproc": os.cpu_count()},
         }
         result["speedup_vs_cpu"] = solves_per_s / (done / tc)
+        result["speedup_vs_cpu_single_thread"] = solves_per_s / (done1 / t1)
 
     print(json.dumps(result), flush=True)
 
