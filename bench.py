@@ -38,6 +38,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "RK45 ensemble solves/sec at tol 1e-8 vs plain GPU path; adjoint grads/sec"
+PLAIN_SAMPLE = 1024  # instances timed on the plain per-instance path
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix peaks are equal)
 PEAK_HBM_GBS = 8000.0
 # algorithmic FLOPs of one RK45 step attempt (fma = 2):
@@ -481,7 +482,7 @@ def main():
         "(up to 20), so late final states differ at O(1) from any other solver's")
 
     if not args.no_plain:
-        m = 1024
+        m = PLAIN_SAMPLE
         log(f"[bench] plain per-instance path, {m} instances")
         plain_sps, pb = plain_path(cx, w, m)
         same = bool(np.array_equal(pb["y_out"].cpu().numpy(),
@@ -514,11 +515,13 @@ def main():
                 "tuning": tnu.as_dict(),
                 "error_vs_tol_1e-13": tight_error(cx, wu, bu["y_out"])}
         if not args.no_plain:
-            plain_u, pbu = plain_path(cx, wu, 1024)
+            mu = PLAIN_SAMPLE
+            plain_u, pbu = plain_path(cx, wu, mu)
             tol8["plain_gpu_solves_per_s"] = plain_u
+            tol8["plain_sample"] = f"first {mu} instances"
             tol8["speedup_vs_plain"] = (n / wall_u) / plain_u
             tol8["identical_to_batched"] = bool(np.array_equal(
-                pbu["y_out"].cpu().numpy(), bu["y_out"][:, :64].cpu().numpy()))
+                pbu["y_out"].cpu().numpy(), bu["y_out"][:, :mu].cpu().numpy()))
         result["uniform_tol_1e-8"] = tol8
         del bu, du
 
