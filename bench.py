@@ -19,6 +19,8 @@ The JSON line also carries:
     stage right-hand sides as FP64 MFMA GEMMs;
   * the other methods and outputs on one shared Lorenz workload: RK23,
     Adams5, path output, the solve_ivp chain, and BDF2;
+  * a captured HIP graph's replay against the eager entry point, on short
+    BDF6 and RK45 solves;
   * the CPU oracle timed on the host cores over a bounded sample.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1]
@@ -105,6 +107,80 @@ def rk_ensemble(cx, w, order=True):
         t_out=b["t_out"].data_ptr(), y_out=b["y_out"].data_ptr(),
         status=b["status"].data_ptr(), counts=b["counts"].data_ptr())
     return b, d
+
+
+def graph_replay(cx):
+    """One captured graph replay against the eager entry point, on short
+    solves where the launch is a visible share of the time: 256 Brusselator
+    instances under BDF6 to t = 0.05 (one wave each), and 4096 Lorenz-63
+    instances under RK45 to t = 0.1.  The eager call takes its workspace from
+    the stream-ordered allocator and launches a setup kernel and the solve
+    kernel; the graph owns its workspace and replays both kernels.  Each
+    side: 50 back-to-back launches on one stream, after 3 warm-up launches;
+    the replay's outputs are checked bit for bit against the eager solve."""
+    from bacon_amd import workloads as W
+    L, torch = cx.L, cx.torch
+    reps = 50
+    out = {}
+
+    def bufs(w, n, dim):
+        return dict(y0=cx.T(w["y0"]), params=cx.T(w["params"]), t_out=cx.zeros(n),
+                    y_out=cx.zeros((dim, n)), status=cx.zeros(n, torch.int32),
+                    counts=cx.zeros((n, 2), torch.int32))
+
+    def desc(b, n, dim, t_end, tol, dt_min):
+        return L.EnsembleDesc(
+            n=n, dim=dim, n_params=3, y0=b["y0"].data_ptr(), params=b["params"].data_ptr(),
+            t0_s=0.0, t_end_s=t_end, tol_s=tol, dt_min_s=dt_min, dt_max_s=0.1,
+            t_out=b["t_out"].data_ptr(), y_out=b["y_out"].data_ptr(),
+            status=b["status"].data_ptr(), counts=b["counts"].data_ptr())
+
+    cases = []
+    nb = 256
+    bb = bufs(W.brusselator(nb), nb, 32)
+    db = desc(bb, nb, 32, 0.05, 1e-7, 1e-12)
+    cases.append(("bdf6_brusselator_256_t0.05", bb, db,
+                  lambda: cx.lib.bg_bdf_solve_device(L.BDF6, L.RHS["brusselator"], L.INTENDED,
+                                                     64, C.byref(db), cx.sh),
+                  lambda g: cx.lib.bg_graph_capture_bdf(L.BDF6, L.RHS["brusselator"],
+                                                        L.INTENDED, 64, C.byref(db),
+                                                        C.byref(g))))
+    nr = 4096
+    br = bufs(W.lorenz_plain(nr), nr, 3)
+    dr = desc(br, nr, 3, 0.1, 1e-6, 1e-6)
+    lc = L.Launch(block=256, blocks_per_cu=2)
+    cases.append(("rk45_lorenz_4096_t0.1", br, dr,
+                  lambda: cx.lib.bg_rk_solve_device(L.RK45, L.RHS["lorenz"], L.INTENDED,
+                                                    C.byref(dr), C.byref(lc), cx.sh),
+                  lambda g: cx.lib.bg_graph_capture_rk(L.RK45, L.RHS["lorenz"], L.INTENDED,
+                                                       C.byref(dr), C.byref(lc), C.byref(g))))
+    for name, b, d, eager, capture in cases:
+        def run_eager():
+            rc = eager()
+            if rc != 0:
+                raise RuntimeError(f"{name}: eager solve returned {rc}")
+        wall_e, kern_e = time_launches(cx, run_eager, reps, 3)
+        ref = b["y_out"].cpu().numpy().copy()
+        g = C.c_void_p()
+        rc = capture(g)
+        if rc != 0:
+            raise RuntimeError(f"{name}: capture returned {rc}")
+        try:
+            def run_graph():
+                r = cx.lib.bg_graph_launch(g, cx.sh)
+                if r != 0:
+                    raise RuntimeError(f"{name}: replay returned {r}")
+            b["y_out"].zero_()
+            wall_g, kern_g = time_launches(cx, run_graph, reps, 3)
+            same = bool(np.array_equal(b["y_out"].cpu().numpy(), ref))
+        finally:
+            cx.lib.bg_graph_destroy(g)
+        out[name] = {"eager_us_per_solve": wall_e * 1e6, "graph_us_per_solve": wall_g * 1e6,
+                     "eager_event_us": kern_e * 1e6, "graph_event_us": kern_g * 1e6,
+                     "replay_identical_to_eager": same}
+    out["method"] = (f"{reps} back-to-back launches per side on one stream; us_per_solve = host "
+                     "wall time / launches; event_us = HIP events around each launch")
+    return out
 
 
 def autotune(cx, d, args):
@@ -345,8 +421,9 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--extras", default="tol8,adjoint,bdf,linear,family",
-                    help="comma list of extra measurements: tol8, adjoint, bdf, linear, family")
+    ap.add_argument("--extras", default="tol8,adjoint,bdf,linear,family,graph",
+                    help="comma list of extra measurements: tol8, adjoint, bdf, linear, "
+                         "family, graph")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the headline ensemble (used for profiling)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -673,6 +750,10 @@ def main():
 
     if "family" in extras:
         result["method_family"] = method_family(cx, args)
+
+    if "graph" in extras:
+        log("[bench] graph replay vs eager launch")
+        result["graph_replay"] = graph_replay(cx)
 
     if not args.no_cpu_baseline:
         import oracle as O
