@@ -143,6 +143,8 @@ SIGNATURES = [
      [C.POINTER(Tuning), C.POINTER(EnsembleDesc), C.c_void_p, C.c_void_p]),
     ("bg_rk_solve_tuned_device", C.c_int,
      [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.POINTER(Tuning), C.c_void_p]),
+    ("bg_schedule_by_counts_device", C.c_int,
+     [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p]),
     ("bg_graph_capture_rk", C.c_int,
      [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.POINTER(Launch),
       C.POINTER(C.c_void_p)]),

@@ -64,6 +64,23 @@ def solve_vjp(rhs: str, y0: torch.Tensor, params: torch.Tensor, obs_t: torch.Ten
     return out
 
 
+def longest_first(counts: torch.Tensor, stream=None) -> torch.Tensor:
+    """Schedule for a later solve of the same instances: the permutation
+    (int32 [n], device) that runs them longest first by the attempts of an
+    earlier solve (`counts` [n, 2] int32: accepted, rejected), computed by
+    bg_schedule_by_counts_device.  In an optimisation loop the previous
+    iteration's counts predict the next one's; pass the result as `order`."""
+    counts = counts.contiguous()
+    n = counts.shape[0]
+    order = torch.empty(n, dtype=torch.int32, device=counts.device)
+    sh = C.c_void_p(stream.cuda_stream if stream is not None
+                    else torch.cuda.current_stream(counts.device).cuda_stream)
+    rc = L.lib().bg_schedule_by_counts_device(_ptr(counts), n, _ptr(order), sh)
+    if rc != 0:
+        raise RuntimeError(f"bg_schedule_by_counts_device returned {rc}")
+    return order
+
+
 class RK45Loss(torch.autograd.Function):
     """loss[i] = sum_k |y_i(t_k) - target_ik|^2 with gradients w.r.t. params and y0."""
 

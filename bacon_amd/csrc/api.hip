@@ -773,6 +773,11 @@ extern "C" int bg_rk_schedule_device(const bg_tuning *tuning, const bg_ensemble_
     return rk_schedule(*tuning, *d, order, (hipStream_t)stream);
 }
 
+extern "C" int bg_schedule_by_counts_device(const bg_counts *counts, uint64_t n, int32_t *order,
+                                            void *stream) {
+    return schedule_by_counts(counts, n, order, (hipStream_t)stream);
+}
+
 extern "C" int bg_rk_solve_tuned_device(int method, int rhs, int semantics,
                                         const bg_ensemble_desc *d, const bg_tuning *tuning,
                                         void *stream) {

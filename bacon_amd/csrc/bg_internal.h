@@ -51,6 +51,7 @@ int rk_schedule(const bg_tuning &tn, const bg_ensemble_desc &d, int32_t *order,
                 hipStream_t stream);
 int rk_solve_tuned(int method, int rhs, int semantics, const bg_ensemble_desc &d,
                    const bg_tuning &tn, hipStream_t stream);
+int schedule_by_counts(const bg_counts *counts, uint64_t n, int32_t *order, hipStream_t stream);
 
 // RK45 for y' = A y + b with one shared A (dim 128), the stage right-hand
 // sides as FP64 MFMA GEMMs over tiles of 16 instances (linear_kernels.hip).

@@ -292,6 +292,46 @@ int rk_schedule(const bg_tuning &tn, const bg_ensemble_desc &d, int32_t *order,
     return schedule(tn, d, order, stream);
 }
 
+// Keys of the measured schedule: an earlier solve's attempts per instance
+// (accepted + rejected, saturating), instance indices beside them.
+__global__ void count_keys_kernel(const bg_counts *__restrict__ counts, uint64_t n,
+                                  uint32_t *__restrict__ keys, int32_t *__restrict__ idx) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bg_counts c = counts[i];
+    const uint32_t a = c.n_accepted + c.n_rejected;
+    keys[i] = a < c.n_accepted ? 0xffffffffu : a;
+    idx[i] = (int32_t)i;
+}
+
+// Longest first by measured work: a stable descending radix sort of the
+// keys, so equal counts keep index order and the permutation is
+// deterministic.
+int schedule_by_counts(const bg_counts *counts, uint64_t n, int32_t *order, hipStream_t stream) {
+    if (n == 0) return BG_OK;
+    if (n > 0x7fffffffull || !counts || !order) return BG_ERR_INVALID_ARGUMENT;
+    size_t sort_bytes = 0;
+    BG_HIP_CHECK(rocprim::radix_sort_pairs_desc(nullptr, sort_bytes, (const uint32_t *)nullptr,
+                                                (uint32_t *)nullptr, (const int32_t *)nullptr,
+                                                (int32_t *)nullptr, (uint32_t)n, 0, 32, stream));
+    const size_t kbytes = ((n * sizeof(uint32_t)) + 255) & ~(size_t)255;
+    char *ws = nullptr;
+    BG_HIP_CHECK(hipMallocAsync((void **)&ws, 3 * kbytes + sort_bytes, stream));
+    uint32_t *keys = (uint32_t *)ws;
+    uint32_t *keys_sorted = (uint32_t *)(ws + kbytes);
+    int32_t *idx = (int32_t *)(ws + 2 * kbytes);
+    void *tmp = ws + 3 * kbytes;
+    count_keys_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, stream>>>(counts, n, keys, idx);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = rocprim::radix_sort_pairs_desc(tmp, sort_bytes, keys, keys_sorted, idx, order,
+                                           (uint32_t)n, 0, 32, stream);
+    // the workspace goes back on every path
+    BG_HIP_CHECK(hipFreeAsync(ws, stream));
+    BG_HIP_CHECK(e);
+    return BG_OK;
+}
+
 int rk_solve_tuned(int method, int rhs, int semantics, const bg_ensemble_desc &din,
                    const bg_tuning &tn, hipStream_t stream) {
     if (rhs == BG_RHS_LINEAR || din.order || din.n < 2)

@@ -274,6 +274,14 @@ int bg_rk_autotune_device(int method, int rhs, int semantics, const bg_ensemble_
 /* The plan's schedule permutation of d's instances into order[n] (device). */
 int bg_rk_schedule_device(const bg_tuning *tuning, const bg_ensemble_desc *d, int32_t *order,
                           void *stream);
+/* Longest-first permutation order[n] (device) from an earlier solve's
+ * per-instance work counts[n] (device; accepted + rejected attempts), e.g.
+ * the previous iteration of an optimisation loop over the same instances.
+ * Stable: equal counts keep index order.  For d->order of the RK ensemble or
+ * of the adjoint's forward pass.  No counterpart in the reference, like the
+ * autotuner. */
+int bg_schedule_by_counts_device(const bg_counts *counts, uint64_t n, int32_t *order,
+                                 void *stream);
 /* Schedule + solve with the plan's launch shape, asynchronous on `stream`.
  * If d->order is set it is used as given. */
 int bg_rk_solve_tuned_device(int method, int rhs, int semantics, const bg_ensemble_desc *d,

@@ -161,3 +161,20 @@ def test_tape_in_lds_and_hbm_agree(gpu, lv):
         assert np.all(g["status"] == 0)
         for key in ("grad_params", "grad_y0", "loss", "obs_y"):
             assert np.array_equal(g[key], base[key]), (k, key)
+
+
+def test_adjoint_in_measured_order_bit_identical(gpu, lv):
+    """The forward pass scheduled longest first by an earlier solve's counts
+    (bacon_amd.adjoint.longest_first) changes no bits."""
+    from bacon_amd.adjoint import longest_first, solve_vjp
+    import torch
+    base = _gpu(lv)
+    first = solve_vjp("lotka_volterra", _t(lv["y0"]), _t(lv["params"]), _t(lv["obs"]),
+                      _t(lv["targets"]), 15.0, TOL)
+    order = longest_first(first["counts"])
+    torch.cuda.synchronize()
+    att = first["counts"].cpu().numpy().astype(np.int64).sum(1)
+    assert np.array_equal(order.cpu().numpy(), np.argsort(-att, kind="stable"))
+    g = _gpu(lv, order=order)
+    for key in ("grad_params", "grad_y0", "loss", "obs_y", "counts", "status"):
+        assert np.array_equal(g[key], base[key]), key

@@ -123,3 +123,50 @@ def test_autotune_rejects_given_order(setup):
     tn = L.Tuning()
     assert L.lib().bg_rk_autotune_device(L.RK45, L.RHS["lorenz"], L.INTENDED, C.byref(d),
                                          C.byref(tn), None) == -1
+
+
+def test_schedule_by_counts_is_a_stable_descending_sort(setup):
+    """bg_schedule_by_counts_device against numpy's stable sort of the same
+    keys: the reference solve's attempts, plus a synthetic set with many
+    ties, a saturating sum and n = 1; n = 0 writes nothing and a missing
+    pointer is rejected."""
+    import torch
+    dev = "cuda:0"
+    att = _np(setup["ref"])["cnt"]
+    rng = np.random.default_rng(7)
+    ties = np.stack([rng.integers(0, 4, 5000), rng.integers(0, 3, 5000)], 1)
+    ties[17] = (0xFFFFFFF0, 0x20)  # sum past 2^32 - 1 counts as the maximum
+    for cnt in (att, ties, att[:1]):
+        n = cnt.shape[0]
+        c = torch.as_tensor(cnt.astype(np.uint32).view(np.int32), device=dev)
+        order = torch.full((n,), -1, dtype=torch.int32, device=dev)
+        assert L.lib().bg_schedule_by_counts_device(c.data_ptr(), n, order.data_ptr(),
+                                                    None) == 0
+        torch.cuda.synchronize()
+        key = np.minimum(cnt[:, 0].astype(np.uint64) + cnt[:, 1].astype(np.uint64),
+                         0xFFFFFFFF)
+        want = np.argsort(-key.astype(np.int64), kind="stable")
+        assert np.array_equal(order.cpu().numpy(), want)
+    order = torch.full((4,), -1, dtype=torch.int32, device=dev)
+    assert L.lib().bg_schedule_by_counts_device(None, 0, order.data_ptr(), None) == 0
+    assert np.all(order.cpu().numpy() == -1)
+    assert L.lib().bg_schedule_by_counts_device(None, 4, order.data_ptr(), None) == -1
+
+
+def test_solve_in_measured_order_bit_identical(setup):
+    """A solve scheduled by the reference solve's own counts returns the
+    same bits per instance."""
+    import torch
+    ref = _np(setup["ref"])
+    c = setup["ref"]["cnt"]
+    order = torch.empty(N, dtype=torch.int32, device="cuda:0")
+    assert L.lib().bg_schedule_by_counts_device(c.data_ptr(), N, order.data_ptr(), None) == 0
+    o = setup["outs"]()
+    d = setup["desc"](o)
+    d.order = order.data_ptr()
+    assert L.lib().bg_rk_solve_device(L.RK45, L.RHS["lorenz"], L.INTENDED, C.byref(d), None,
+                                      None) == 0
+    torch.cuda.synchronize()
+    got = _np(o)
+    for k in ref:
+        assert np.array_equal(got[k], ref[k]), k
