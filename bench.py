@@ -784,36 +784,46 @@ def main():
         import oracle as O
         threads = min(os.cpu_count() or 1, 16)
         log(f"[bench] CPU oracle baseline ({threads} threads, ~{args.cpu_seconds}s)")
-        done, chunk = 0, 512
-        tc = time.perf_counter()
-        while time.perf_counter() - tc < args.cpu_seconds and done < n:
-            s = W.lorenz_batched(chunk, first=done)
-            O.solve_ensemble(O.RK45, "lorenz", s["y0"], s["params"], s["t_end"], s["tol"],
-                             dt_min=w["dt_min"], dt_max=w["dt_max"], semantics=O.INTENDED,
-                             profile=O.PROFILE_BACON, threads=threads)
-            done += chunk
-        tc = time.perf_counter() - tc
+
+        def cpu_runs(nthreads, seconds, chunk):
+            """Three runs over one sample: the first solves as many chunks
+            of the workload as fit in seconds / 3, the other two the same
+            instances again.  Returns (instances, [solves/s per run])."""
+            def solve(first):
+                sm = W.lorenz_batched(chunk, first=first)
+                O.solve_ensemble(O.RK45, "lorenz", sm["y0"], sm["params"], sm["t_end"],
+                                 sm["tol"], dt_min=w["dt_min"], dt_max=w["dt_max"],
+                                 semantics=O.INTENDED, profile=O.PROFILE_BACON,
+                                 threads=nthreads)
+            done, t = 0, time.perf_counter()
+            while time.perf_counter() - t < seconds / 3 and done < n:
+                solve(done)
+                done += chunk
+            rates = [done / (time.perf_counter() - t)]
+            for _ in range(2):
+                t = time.perf_counter()
+                for first in range(0, done, chunk):
+                    solve(first)
+                rates.append(done / (time.perf_counter() - t))
+            return done, rates
+
+        done, rates = cpu_runs(threads, args.cpu_seconds, 512)
         # bacon itself is single-threaded: the same oracle on one thread, on a
         # shorter sample of the same instances
-        done1, chunk1 = 0, 64
-        t1 = time.perf_counter()
-        while time.perf_counter() - t1 < args.cpu_seconds / 4 and done1 < n:
-            s = W.lorenz_batched(chunk1, first=done1)
-            O.solve_ensemble(O.RK45, "lorenz", s["y0"], s["params"], s["t_end"], s["tol"],
-                             dt_min=w["dt_min"], dt_max=w["dt_max"], semantics=O.INTENDED,
-                             profile=O.PROFILE_BACON, threads=1)
-            done1 += chunk1
-        t1 = time.perf_counter() - t1
+        done1, rates1 = cpu_runs(1, args.cpu_seconds / 4, 64)
+        rate, rate1 = float(np.median(rates)), float(np.median(rates1))
         result["cpu_baseline"] = {
-            "value": done / tc, "unit": "solves/s", "cores": threads, "kind": "port",
+            "value": rate, "unit": "solves/s", "cores": threads, "kind": "port",
             "sample": f"first {done} instances of the same workload, C oracle "
-                      f"(bacon arithmetic, -O2), OpenMP over {threads} threads, {tc:.1f} s",
-            "single_thread": {"value": done1 / t1, "unit": "solves/s",
-                              "sample": f"first {done1} instances, one thread, {t1:.1f} s"},
+                      f"(bacon arithmetic, -O2), OpenMP over {threads} threads; median of 3 "
+                      f"runs over that sample, {args.cpu_seconds:.0f} s in all",
+            "runs": rates,
+            "single_thread": {"value": rate1, "unit": "solves/s", "runs": rates1,
+                              "sample": f"first {done1} instances, one thread, median of 3 runs"},
             "host": {"cpu_model": cpu_model(), "nproc": os.cpu_count()},
         }
-        result["speedup_vs_cpu"] = solves_per_s / (done / tc)
-        result["speedup_vs_cpu_single_thread"] = solves_per_s / (done1 / t1)
+        result["speedup_vs_cpu"] = solves_per_s / rate
+        result["speedup_vs_cpu_single_thread"] = solves_per_s / rate1
 
     print(json.dumps(result), flush=True)
 
