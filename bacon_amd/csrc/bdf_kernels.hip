@@ -22,6 +22,11 @@ namespace bg {
 
 static constexpr uint32_t kBdfDefaultMaxSteps = 1000000u;
 
+// Per-wave scratch of the whole-wave (32-component) layout, in doubles: the
+// lower-order residual's finite-difference Jacobian [32 columns][32 rows],
+// then per lane that residual at y_n and the guess after the column loop.
+static constexpr uint32_t kBdfScratchDoubles = 32 * 32 + 2 * 64;
+
 // ---------------------------------------------------------------------------
 // cross-lane moves without LDS traffic (gfx950 lane layouts, measured):
 //   DPP row_ror:1 (0x121): lane i <- lane (i - 1) mod 16 within its row of 16
@@ -265,6 +270,7 @@ struct BdfLane {
     Grp<W> grp;
     int h = 0;    // half of the wave (SPLIT = 2)
     double *lds;  // this group's W x (W+1) transpose buffer, then 2 scalars
+    double *scr = nullptr;  // SPLIT = 2: this wave's kBdfScratchDoubles
     // instance scalars (replicated across the group)
     double t, dt, end, tol, dt_min, dt_max;
     int yield_memory;
@@ -323,19 +329,25 @@ struct BdfLane {
         return BG_OK;
     }
 
-    // residual closures bdf.rs:548-575 (lane g's component)
+    // residual closures bdf.rs:548-575 (lane g's component), given
+    // fx = f(t_n, x)_g
     template <bool LOWER>
-    __device__ __forceinline__ bool residual(double tt, double x, double &g) const {
-        double fx;
-        if (!f(tt, x, fx)) return false;
+    __device__ __forceinline__ double combine(double fx, double x) const {
         constexpr bool own = LOWER && Sem::lower_own_hist;
-        g = (-fx) * dt * (LOWER ? C::lo(0) : C::hi(0));
+        double g = (-fx) * dt * (LOWER ? C::lo(0) : C::hi(0));
 #pragma unroll
         for (int ind = 1; ind < O; ++ind) {
             const double c = own ? C::lo(ind) : C::hi(ind);
             if (c != 0.0) g = fma(hist[O - ind], c, g);
         }
-        g = g + x;
+        return g + x;
+    }
+
+    template <bool LOWER>
+    __device__ __forceinline__ bool residual(double tt, double x, double &g) const {
+        double fx;
+        if (!f(tt, x, fx)) return false;
+        g = combine<LOWER>(fx, x);
         return true;
     }
 
@@ -475,16 +487,37 @@ struct BdfLane {
         return ok;
     }
 
-    // bdf.rs:414-475; on success `guess` holds the corrected value
+    // bdf.rs:414-475; on success `guess` holds the corrected value.
+    // SPLIT = 2: both secants of a step start from guess = y_n at the same
+    // time, and the column loop's restores are the same in both, so every
+    // f(t_n, x +/- dt e_c) the lower-order loop would evaluate equals one of
+    // the higher-order loop's.  The higher-order call therefore also forms
+    // the lower-order residual's columns, its residual at y_n and the
+    // restored guess from those same values, in the same operations, and
+    // leaves them in the wave's scratch; the lower-order call reads them
+    // back instead of evaluating the right-hand side again.
     template <bool LOWER>
     __device__ __forceinline__ int secant(double &guess) {
         const double tt = t + Sem::theta * dt;
-        guess = y;
+        const double denom = 1.0 / (2.0 * dt);
+        const int lane = grp.g + h * W;
         double F;
-        if (!residual<LOWER>(tt, guess, F)) return BG_USER_ERROR;
+        if constexpr (SPLIT == 2 && LOWER) {
+            // the columns and values left by secant<false> of this step
+            __threadfence_block();
+            F = scr[W * W + lane];
+            guess = scr[W * W + 64 + lane];
+#pragma unroll
+            for (int j = 0; j < WJ; ++j)
+                lds[grp.g * (W + 1) + h * WJ + j] = scr[(h * WJ + j) * W + grp.g];
+        } else {
+        guess = y;
+        double fx0;
+        if (!f(tt, guess, fx0)) return BG_USER_ERROR;
+        F = combine<LOWER>(fx0, guess);
+        if constexpr (SPLIT == 2) scr[W * W + lane] = combine<true>(fx0, guess);
         // bdf.rs:390-411: columns (G(x + dt e_c) +/- G(x - dt e_c)) / (2 dt);
         // lane g stores J(g, c) into row g of the group's LDS buffer
-        const double denom = 1.0 / (2.0 * dt);
 #pragma unroll 1
         for (int c = 0; c < W; ++c) {
             const bool mine = (grp.g == c);
@@ -492,12 +525,17 @@ struct BdfLane {
                 // half 0 evaluates at x + dt e_c, half 1 at (x + dt e_c) - 2 dt e_c
                 const double xp = guess + dt, xm = xp - 2.0 * dt;
                 const double xg = mine ? (h == 0 ? xp : xm) : guess;
-                double r;
-                if (!residual<LOWER>(tt, xg, r)) return BG_USER_ERROR;
+                double fx;
+                if (!f(tt, xg, fx)) return BG_USER_ERROR;
+                const double r = combine<false>(fx, xg);
                 const double other = xor32_f64(r);
                 const double above = h == 0 ? r : other, below = h == 0 ? other : r;
+                const double rl = combine<true>(fx, xg);
+                const double other_l = xor32_f64(rl);
+                const double above_l = h == 0 ? rl : other_l, below_l = h == 0 ? other_l : rl;
                 if (mine) guess = xm + dt;  // the reference's restore (bdf.rs:405)
                 lds[grp.g * (W + 1) + c] = (above + Sem::fd_sign * below) * denom;
+                if (h == 0) scr[c * W + grp.g] = (above_l + Sem::fd_sign * below_l) * denom;
             } else {
                 if (mine) guess += dt;
                 double above, below;
@@ -507,6 +545,8 @@ struct BdfLane {
                 if (mine) guess += dt;
                 lds[grp.g * (W + 1) + c] = (above + Sem::fd_sign * below) * denom;
             }
+        }
+        if constexpr (SPLIT == 2) scr[W * W + 64 + lane] = guess;
         }
         __builtin_amdgcn_wave_barrier();
         ++n_lu;
@@ -662,7 +702,8 @@ struct BdfLane {
 // measured 20 % faster than the 2 that fit without spilling (4: slower).
 template <class RHS, int O, class Sem, class LuT, int SPLIT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
-    bdf_ensemble_kernel(const volatile bg_ensemble_desc *dv, uint32_t *__restrict__ counter) {
+    bdf_ensemble_kernel(const volatile bg_ensemble_desc *dv, uint32_t *__restrict__ counter,
+                        double *__restrict__ scratch) {
     constexpr int W = RHS::D;
     constexpr int GW = W * SPLIT;  // lanes per instance
     constexpr int GPW = 64 / GW;   // groups per wave
@@ -673,6 +714,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
     L.grp.g = lane % W;
     L.h = (lane % GW) / W;
     L.lds = lds_all[grp_id];
+    // one wave per block; SPLIT = 2 holds one instance per wave
+    if constexpr (SPLIT == 2) L.scr = scratch + (uint64_t)blockIdx.x * kBdfScratchDoubles;
     const uint64_t max_att = dv->max_attempts;
     const uint32_t cap =
         (uint32_t)((max_att && max_att < kBdfDefaultMaxSteps) ? max_att : kBdfDefaultMaxSteps);
@@ -884,8 +927,42 @@ bool bdf_supported(int method, int rhs, uint32_t dim) {
            BG_OK;
 }
 
+// Persistent grid (one wave per block, as many as are resident) and the
+// launch workspace: counter and descriptor copy, then for the whole-wave
+// layout kBdfScratchDoubles per block.
+template <class R, int O, class S, class F>
+static int bdf_launch_shape(const bg_ensemble_desc &d, uint64_t &grid, size_t &bytes) {
+    constexpr int W = R::D;
+    constexpr int SP = W == 32 ? 2 : 1;  // a 32-component instance takes the whole wave
+    constexpr int GPW = 64 / (W * SP);
+    int dev = 0;
+    BG_HIP_CHECK(hipGetDevice(&dev));
+    int per_cu = 0;
+    BG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, bdf_ensemble_kernel<R, O, S, F, SP>, 64, 0));
+    if (per_cu < 1) per_cu = 1;
+    grid = (uint64_t)per_cu * (uint64_t)device_cu_count(dev);
+    const uint64_t need = (d.n + GPW - 1) / GPW;
+    if (grid > need) grid = need;
+    bytes = kLaunchWorkspaceBytes + (SP == 2 ? grid * kBdfScratchDoubles * sizeof(double) : 0);
+    return BG_OK;
+}
+
+int bdf_workspace_bytes(int method, int rhs, int semantics, int lu_fp32, const bg_ensemble_desc &d,
+                        size_t *bytes) {
+    return visit_bdf(method, rhs, semantics, d.dim, lu_fp32,
+                     [&](auto rtag, auto otag, auto stag, auto ftag) -> int {
+                         using R = typename decltype(rtag)::type;
+                         using S = typename decltype(stag)::type;
+                         using F = typename decltype(ftag)::type;
+                         constexpr int O = decltype(otag)::value;
+                         uint64_t grid = 0;
+                         return bdf_launch_shape<R, O, S, F>(d, grid, *bytes);
+                     });
+}
+
 int bdf_launch_ensemble(int method, int rhs, int semantics, int lu_fp32, const bg_ensemble_desc &d,
-                        hipStream_t stream, void *ws_in) {
+                        hipStream_t stream, void *ws_in, size_t ws_bytes) {
     return visit_bdf(method, rhs, semantics, d.dim, lu_fp32,
                      [&](auto rtag, auto otag, auto stag, auto ftag) -> int {
                          using R = typename decltype(rtag)::type;
@@ -896,28 +973,21 @@ int bdf_launch_ensemble(int method, int rhs, int semantics, int lu_fp32, const b
                          if (d.n > 0x7fffffffull || !d.y0) return BG_ERR_INVALID_ARGUMENT;
                          if (R::P > 0 && (!d.params || d.n_params < (uint32_t)R::P))
                              return BG_ERR_INVALID_ARGUMENT;
-                         constexpr int W = R::D;
                          // a 32-component instance takes the whole wave
-                         constexpr int SP = W == 32 ? 2 : 1;
-                         constexpr int GPW = 64 / (W * SP);
-                         int dev = 0;
-                         BG_HIP_CHECK(hipGetDevice(&dev));
-                         int per_cu = 0;
-                         BG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                             &per_cu, bdf_ensemble_kernel<R, O, S, F, SP>, 64, 0));
-                         if (per_cu < 1) per_cu = 1;
-                         uint64_t grid = (uint64_t)per_cu * (uint64_t)device_cu_count(dev);
-                         const uint64_t need = (d.n + GPW - 1) / GPW;
-                         if (grid > need) grid = need;
+                         constexpr int SP = R::D == 32 ? 2 : 1;
+                         uint64_t grid = 0;
+                         size_t bytes = 0;
+                         const int rc = bdf_launch_shape<R, O, S, F>(d, grid, bytes);
+                         if (rc != BG_OK) return rc;
+                         if (ws_in && ws_bytes < bytes) return BG_ERR_INVALID_ARGUMENT;
                          char *ws = (char *)ws_in;
-                         if (!ws_in)
-                             BG_HIP_CHECK(
-                                 hipMallocAsync((void **)&ws, kLaunchWorkspaceBytes, stream));
+                         if (!ws_in) BG_HIP_CHECK(hipMallocAsync((void **)&ws, bytes, stream));
                          uint32_t *counter = (uint32_t *)ws;
                          bg_ensemble_desc *ddesc = (bg_ensemble_desc *)(ws + 256);
+                         double *scratch = SP == 2 ? (double *)(ws + kLaunchWorkspaceBytes) : nullptr;
                          bdf_setup_kernel<<<1, 64, 0, stream>>>(counter, ddesc, d);
                          bdf_ensemble_kernel<R, O, S, F, SP>
-                             <<<(uint32_t)grid, 64, 0, stream>>>(ddesc, counter);
+                             <<<(uint32_t)grid, 64, 0, stream>>>(ddesc, counter, scratch);
                          const hipError_t launched = hipGetLastError();
                          if (!ws_in) BG_HIP_CHECK(hipFreeAsync(ws, stream));
                          BG_HIP_CHECK(launched);

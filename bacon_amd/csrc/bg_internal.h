@@ -89,8 +89,12 @@ struct BdfPathState {
     double p[8];
 };
 bool bdf_supported(int method, int rhs, uint32_t dim);
+// ws == nullptr: the launcher allocates the workspace; else ws holds
+// ws_bytes, at least bdf_workspace_bytes() for the same arguments.
 int bdf_launch_ensemble(int method, int rhs, int semantics, int lu_fp32, const bg_ensemble_desc &d,
-                        hipStream_t stream, void *ws = nullptr);
+                        hipStream_t stream, void *ws = nullptr, size_t ws_bytes = 0);
+int bdf_workspace_bytes(int method, int rhs, int semantics, int lu_fp32, const bg_ensemble_desc &d,
+                        size_t *bytes);
 int bdf_launch_path(int method, int rhs, int semantics, uint32_t dim, BdfPathState *state,
                     double *path_t, double *path_y, uint32_t cap, hipStream_t stream);
 

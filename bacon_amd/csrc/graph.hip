@@ -23,14 +23,14 @@ struct bg_graph {
 namespace bg {
 
 template <class F>
-static int capture(F &&launch, bg_graph **out) {
+static int capture(size_t ws_bytes, F &&launch, bg_graph **out) {
     if (!out) return BG_ERR_INVALID_ARGUMENT;
     *out = nullptr;
     bg_graph *g = new bg_graph;
     hipStream_t s = nullptr;
     int rc = BG_OK;
     do {
-        if (hipMalloc(&g->ws, kLaunchWorkspaceBytes) != hipSuccess) {
+        if (hipMalloc(&g->ws, ws_bytes) != hipSuccess) {
             rc = BG_ERR_HIP;
             break;
         }
@@ -75,6 +75,7 @@ extern "C" int bg_graph_capture_rk(int method, int rhs, int semantics, const bg_
     if (launch) lc = *launch;
     const bg_ensemble_desc dd = *d;
     return capture(
+        kLaunchWorkspaceBytes,
         [&](hipStream_t s, void *ws) {
             return rk_launch_ensemble(method, rhs, semantics, dd, lc, s, ws);
         },
@@ -87,9 +88,13 @@ extern "C" int bg_graph_capture_bdf(int method, int rhs, int semantics, int lu_b
     if (lu_bits != 64 && lu_bits != 32) return BG_ERR_INVALID_ARGUMENT;
     if (d->n == 0) return BG_ERR_INVALID_ARGUMENT;
     const bg_ensemble_desc dd = *d;
+    size_t bytes = 0;
+    const int rc = bdf_workspace_bytes(method, rhs, semantics, lu_bits == 32, dd, &bytes);
+    if (rc != BG_OK) return rc;
     return capture(
+        bytes,
         [&](hipStream_t s, void *ws) {
-            return bdf_launch_ensemble(method, rhs, semantics, lu_bits == 32, dd, s, ws);
+            return bdf_launch_ensemble(method, rhs, semantics, lu_bits == 32, dd, s, ws, bytes);
         },
         out);
 }

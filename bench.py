@@ -56,7 +56,12 @@ FLOPS_VJP_STEP_LV = (6 * 7 + 2 * 2 * 15 + 6 * 2) + 6 * 20 + 2 * 2 * 15 + 2 * 2 *
 BRUSS_N = 32
 F_RHS_BRUSS = 16 * 19
 F_RESID = F_RHS_BRUSS + 15 * BRUSS_N
-FLOPS_SECANT_SETUP = (1 + 2 * BRUSS_N) * F_RESID + 2 * BRUSS_N ** 3 + 2 * BRUSS_N ** 2
+# Per Jacobian inverse: residual at the guess and the 2N finite-difference
+# columns, then the inverse.  A step's two secants start from the same point,
+# so their right-hand-side evaluations are made once and shared: each inverse
+# carries half of them (the residual combinations are made for both).
+FLOPS_SECANT_SETUP = ((1 + 2 * BRUSS_N) * (F_RHS_BRUSS // 2 + 15 * BRUSS_N)
+                      + 2 * BRUSS_N ** 3 + 2 * BRUSS_N ** 2)
 FLOPS_BROYDEN_ITER = F_RESID + 5 * 2 * BRUSS_N ** 2 + 6 * BRUSS_N
 # linear y' = A y + b, N = 128, one shared A: RHS = matvec (2 N^2) + b (N)
 LIN_N = 128
