@@ -24,8 +24,11 @@
 //    the 4 lanes of the same column (shuffles), and over the 4 waves (LDS) in
 //    a fixed order, so every lane holding the column computes the same
 //    controller decision.  The controller is RkLane's (squared-norm form).
-//  * Tiles are taken from a global counter; a tile runs until all of its
-//    instances are done.
+//  * Instances are taken per column from a global counter: a column whose
+//    instance finishes stores it and takes the next, so the 16 columns of a
+//    GEMM stay filled until the counter runs out.  A column's arithmetic
+//    does not depend on the other columns, so results do not depend on
+//    which column or when an instance runs.
 #include "bg_device.h"
 #include "bg_internal.h"
 
@@ -74,55 +77,101 @@ __global__ void __launch_bounds__(256)
     const uint32_t max_att =
         (uint32_t)((d.max_attempts && d.max_attempts < kLinMaxAttempts) ? d.max_attempts
                                                                        : kLinMaxAttempts);
+    // Column state.  A column whose instance finishes stores it and takes
+    // the next one at the top of the following iteration, so no column
+    // rides along idle while the work counter lasts.
+    double y[2][4], K[S][2][4];
+    double t = 0, dt = 0, end = 0, tol = 1, dt_min = 0, dt_max = 0;
+    int status = BG_OK;  // -1 running; >= 0 finished (or no instance)
+    bool valid = false;  // the column holds an instance not yet stored
+    uint64_t inst = 0;
+    uint32_t attempts = 0, nacc = 0, nrej = 0;
+    bool exhausted = false;  // block-uniform: the counter has passed n
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) y[mt][r] = 0.0;
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) K[i][mt][r] = 0.0;
+
     for (;;) {
-        if (tid == 0) tile_base = atomicAdd(counter, (uint32_t)LT);
-        __syncthreads();
-        const uint64_t base = tile_base;
-        __syncthreads();
-        if (base >= n) break;
-        const uint64_t pos = base + col;
-        const bool valid = pos < n;
-        const uint64_t inst = valid ? (d.order ? (uint64_t)d.order[pos] : pos) : 0;
-        // ---- load the instance column
-        double y[2][4], K[S][2][4];
-        double t = 0, dt = 0, end = 0, tol = 1, dt_min = 0, dt_max = 0;
-        if (valid) {
+        // ---- store finished columns
+        if (valid && status >= 0) {
 #pragma unroll
             for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
-                    y[mt][r] = d.y0[(uint64_t)(LRW * w + 16 * mt + 4 * r + q) * n + inst];
-            t = d.t0 ? d.t0[inst] : d.t0_s;
-            end = d.t_end ? d.t_end[inst] : d.t_end_s;
-            tol = d.tol ? d.tol[inst] : d.tol_s;
-            dt_min = d.dt_min ? d.dt_min[inst] : d.dt_min_s;
-            dt_max = d.dt_max ? d.dt_max[inst] : d.dt_max_s;
-            dt = (dt_max + dt_min) * 0.5;  // rk.rs:315
-        } else {
-#pragma unroll
-            for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) y[mt][r] = 0.0;
+                    if (d.y_out) d.y_out[(uint64_t)(LRW * w + 16 * mt + 4 * r + q) * n + inst] = y[mt][r];
+            if (w == 0 && q == 0) {
+                if (d.t_out) d.t_out[inst] = t;
+                if (d.status) d.status[inst] = status;
+                if (d.counts) {
+                    bg_counts c;
+                    c.n_accepted = nacc;
+                    c.n_rejected = nrej;
+                    d.counts[inst] = c;
+                }
+            }
+            valid = false;
         }
+        // ---- refill idle columns from the work counter
+        if (!exhausted) {
+            red[w][col] = status >= 0 ? 1.0 : 0.0;
+            __syncthreads();
+            uint32_t idle = 0;
 #pragma unroll
-        for (int i = 0; i < S; ++i)
+            for (int c = 0; c < LT; ++c) idle |= (red[0][c] != 0.0 ? 1u : 0u) << c;
+            const uint32_t cnt = (uint32_t)__popc(idle);
+            if (tid == 0 && cnt) tile_base = atomicAdd(counter, cnt);
+            __syncthreads();
+            if (cnt) {
+                const uint64_t base = tile_base;
+                if (base + cnt >= n) exhausted = true;
+                if ((idle >> col) & 1u) {
+                    const uint64_t pos = base + (uint64_t)__popc(idle & ((1u << col) - 1u));
+                    if (pos < n) {
+                        valid = true;
+                        inst = d.order ? (uint64_t)d.order[pos] : pos;
 #pragma unroll
-            for (int mt = 0; mt < 2; ++mt)
+                        for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) K[i][mt][r] = 0.0;  // rk.rs:323-327
-        int status = -1;  // -1 running; >= 0 final status
-        if (!valid || t >= end) status = BG_OK;
-        uint32_t attempts = 0, nacc = 0, nrej = 0;
-
-        for (;;) {
-            // tile finished when every column is
+                            for (int r = 0; r < 4; ++r)
+                                y[mt][r] = d.y0[(uint64_t)(LRW * w + 16 * mt + 4 * r + q) * n + inst];
+                        t = d.t0 ? d.t0[inst] : d.t0_s;
+                        end = d.t_end ? d.t_end[inst] : d.t_end_s;
+                        tol = d.tol ? d.tol[inst] : d.tol_s;
+                        dt_min = d.dt_min ? d.dt_min[inst] : d.dt_min_s;
+                        dt_max = d.dt_max ? d.dt_max[inst] : d.dt_max_s;
+                        dt = (dt_max + dt_min) * 0.5;  // rk.rs:315
+#pragma unroll
+                        for (int i = 0; i < S; ++i)
+#pragma unroll
+                            for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                                for (int r = 0; r < 4; ++r) K[i][mt][r] = 0.0;  // rk.rs:323-327
+                        attempts = nacc = nrej = 0;
+                        status = t >= end ? BG_OK : -1;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        {
+            // any column running?
             red[w][col] = status < 0 ? 1.0 : 0.0;
             __syncthreads();
             double running = 0.0;
 #pragma unroll
             for (int c = 0; c < LT; ++c) running += red[0][c];
             __syncthreads();
-            if (running == 0.0) break;
+            if (running == 0.0) {
+                if (exhausted) break;
+                continue;  // every new instance started at its end: store, refill
+            }
 
             const bool active = status < 0;
             if (active) {
@@ -221,24 +270,6 @@ __global__ void __launch_bounds__(256)
                     if (t >= end) status = BG_OK;  // rk.rs:362-364 on the next call
                 } else {
                     ++nrej;
-                }
-            }
-        }
-        // ---- store the tile's columns
-        if (valid) {
-#pragma unroll
-            for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    if (d.y_out) d.y_out[(uint64_t)(LRW * w + 16 * mt + 4 * r + q) * n + inst] = y[mt][r];
-            if (w == 0 && q == 0) {
-                if (d.t_out) d.t_out[inst] = t;
-                if (d.status) d.status[inst] = status;
-                if (d.counts) {
-                    bg_counts c;
-                    c.n_accepted = nacc;
-                    c.n_rejected = nrej;
-                    d.counts[inst] = c;
                 }
             }
         }

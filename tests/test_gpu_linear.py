@@ -68,6 +68,26 @@ def test_linear_per_instance_ends_and_tolerances(gpu, orc):
         assert np.abs(g["y"][:, i] - r["y"]).max() <= bar, (i, te, tol)
 
 
+def test_linear_column_refill_uneven_ends(gpu):
+    """12,288 instances, more than the resident tiles' 16 columns hold, with
+    end times uniform in [0.5, 5]: columns take new instances as theirs
+    finish (one 16-column GEMM carries instances that started at different
+    times).  Every instance lands on its end time and meets
+    2 tol max(1, |y|) against the closed form."""
+    n = 12288
+    w = W.linear_shared(n)
+    w["t_end"] = np.random.default_rng(11).uniform(0.5, 5.0, n)
+    g = ivp.solve_ensemble(L.RK45, w)
+    assert np.all(g["status"] == 0)
+    assert np.array_equal(g["t"], w["t_end"])
+    xs = np.linalg.solve(w["A"], -w["b"])
+    z = w["Q"].T @ (w["y0"] - xs[:, None])
+    ye = xs[:, None] + w["Q"] @ (np.exp(-w["D"][:, None] * w["t_end"][None, :]) * z)
+    bar = 2e-9 * np.maximum(1.0, np.abs(ye).max(axis=0))
+    err = np.abs(g["y"] - ye).max(axis=0)
+    assert np.all(err <= bar), (int(np.argmax(err / bar)), float((err / bar).max()))
+
+
 def test_linear_as_built_first_attempts(gpu, orc):
     """Reference tableau as shipped (transposed, rk.rs:266 with nalgebra's
     column-major fill): it does not converge on this problem, so compare the
