@@ -59,6 +59,12 @@ __device__ __forceinline__ double xor16_f64(double v) {
     const uint32_t lo = xor16_u32((uint32_t)b), hi = xor16_u32((uint32_t)(b >> 32));
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
+// value of lane l ^ 32, one 32-bit swap
+__device__ __forceinline__ float xor32_f32(float v) {
+    const uint32_t b = __builtin_bit_cast(uint32_t, v);
+    const auto p = __builtin_amdgcn_permlane32_swap(b, b, false, false);
+    return __builtin_bit_cast(float, __lane_id() < 32 ? p[1] : p[0]);
+}
 // value of lane l ^ 32
 __device__ __forceinline__ double xor32_f64(double v) {
     const uint64_t b = __builtin_bit_cast(uint64_t, v);
@@ -69,6 +75,9 @@ __device__ __forceinline__ double xor32_f64(double v) {
     const uint32_t l = low ? lo[1] : lo[0], h = low ? hi[1] : hi[0];
     return __builtin_bit_cast(double, ((uint64_t)h << 32) | l);
 }
+
+__device__ __forceinline__ double xor32_lu(double v) { return xor32_f64(v); }
+__device__ __forceinline__ float xor32_lu(float v) { return xor32_f32(v); }
 
 // ---------------------------------------------------------------------------
 // group collectives (every lane of the group active)
@@ -155,6 +164,14 @@ __device__ __forceinline__ uint32_t pivot_key32(double a, bool eligible, int g) 
     if (!eligible || !(a == a)) return 0u;
     const uint32_t hi = (uint32_t)(__builtin_bit_cast(uint64_t, a) >> 32) & 0x7fffffffu;
     return (hi & ~31u) | (uint32_t)(31 - g);
+}
+// The FP32 inverse's key straight from the float: |a|'s exponent and 23
+// mantissa bits, the 5 lowest replaced by 31 - g (the largest |a| wins up
+// to 2^-18 relative); |a| below 2^-144 counts as zero.
+__device__ __forceinline__ uint32_t pivot_key32(float a, bool eligible, int g) {
+    if (!eligible || !(a == a)) return 0u;
+    const uint32_t m = __builtin_bit_cast(uint32_t, a) & 0x7fffffffu;
+    return (m & ~31u) | (uint32_t)(31 - g);
 }
 __device__ __forceinline__ int pivot_lane32(uint32_t key) { return 31 - (int)(key & 31u); }
 __device__ __forceinline__ bool pivot_nonzero32(uint32_t key) { return (key & ~31u) != 0u; }
@@ -271,6 +288,15 @@ struct BdfLane {
     int h = 0;    // half of the wave (SPLIT = 2)
     double *lds;  // this group's W x (W+1) transpose buffer, then 2 scalars
     double *scr = nullptr;  // SPLIT = 2: this wave's kBdfScratchDoubles
+    float *prow_f = nullptr;  // SPLIT = 2, FP32 inverse: pivot row and 1 / pivot
+
+    // The elimination's pivot row (W entries) and 1 / pivot (entry W), in
+    // the inverse's arithmetic type: FP64 uses the group's LDS buffer (its
+    // row 0, free during the elimination), FP32 a float buffer of its own.
+    __device__ __forceinline__ LuT *pivot_row() const {
+        if constexpr (sizeof(LuT) == sizeof(double)) return lds;
+        else return prow_f;
+    }
     // instance scalars (replicated across the group)
     double t, dt, end, tol, dt_min, dt_max;
     int yield_memory;
@@ -392,13 +418,14 @@ struct BdfLane {
         bool used = false, ok = true;
         int my_col = 0;
         LuT my_inv = (LuT)1;
+        LuT *const prow = pivot_row();
 #pragma unroll
         for (int k = 0; k < W; ++k) {
             const int hk = k / WJ, kk = k % WJ;
             const LuT mk = b[kk];
-            const LuT other = (LuT)xor32_f64((double)mk);
+            const LuT other = xor32_lu(mk);
             const LuT agk = (h == hk) ? mk : other;
-            const uint32_t key = grp.max_u32(pivot_key32((double)agk, !used, grp.g));
+            const uint32_t key = grp.max_u32(pivot_key32(agk, !used, grp.g));
             if (!pivot_nonzero32(key)) ok = false;
             const bool piv = (pivot_lane32(key) == grp.g);
             if (piv) {
@@ -406,16 +433,16 @@ struct BdfLane {
                 my_col = k;
                 my_inv = (LuT)1 / agk;
 #pragma unroll
-                for (int j = 0; j < WJ; ++j) lds[h * WJ + j] = (double)b[j];
-                if (h == 0) lds[W] = (double)my_inv;
+                for (int j = 0; j < WJ; ++j) prow[h * WJ + j] = b[j];
+                if (h == 0) prow[W] = my_inv;
                 // column k of the inverse: pivot * (1 / pivot)
                 if (h == hk) b[kk] = (LuT)1;
             }
             __builtin_amdgcn_wave_barrier();
             if (!piv) {
-                const LuT f = agk * (LuT)lds[W];
+                const LuT f = agk * prow[W];
 #pragma unroll
-                for (int j = 0; j < WJ; ++j) b[j] = fma(-f, (LuT)lds[h * WJ + j], b[j]);
+                for (int j = 0; j < WJ; ++j) b[j] = fma(-f, prow[h * WJ + j], b[j]);
                 // column k of the inverse: -a(g, k) / pivot
                 if (h == hk) b[kk] = -f;
             }
@@ -716,6 +743,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
     L.lds = lds_all[grp_id];
     // one wave per block; SPLIT = 2 holds one instance per wave
     if constexpr (SPLIT == 2) L.scr = scratch + (uint64_t)blockIdx.x * kBdfScratchDoubles;
+    if constexpr (SPLIT == 2 && sizeof(LuT) == sizeof(float)) {
+        __shared__ float prow_all[W + 2];
+        L.prow_f = prow_all;
+    }
     const uint64_t max_att = dv->max_attempts;
     const uint32_t cap =
         (uint32_t)((max_att && max_att < kBdfDefaultMaxSteps) ? max_att : kBdfDefaultMaxSteps);
