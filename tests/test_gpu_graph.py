@@ -99,6 +99,28 @@ def test_bdf_graph_replays(gpu):
         lib.bg_graph_destroy(g)
 
 
+def test_bdf_fp32_graph_replays(gpu):
+    """The FP32 Jacobian inverse under capture: its float pivot-row buffer
+    and the graph-owned scratch of shared finite-difference columns give
+    replays bit-identical to eager FP32 solves."""
+    import torch
+    n = 256
+    w = W.brusselator(n)
+    b = _bufs(w, n, 32)
+    d = _desc(b, n, 32, 3, 10.0, 1e-7, 1e-12, 0.1)
+    lib = L.lib()
+    g = C.c_void_p()
+    assert lib.bg_graph_capture_bdf(L.BDF6, L.RHS["brusselator"], L.INTENDED, 32, C.byref(d),
+                                    C.byref(g)) == 0
+    try:
+        eager = lambda dd: lib.bg_bdf_solve_device(L.BDF6, L.RHS["brusselator"], L.INTENDED, 32,
+                                                   C.byref(dd), None)
+        w2 = W.brusselator(n, first=n)
+        _check_replays(b, d, eager, g, torch.as_tensor(w2["y0"], device="cuda:0"))
+    finally:
+        lib.bg_graph_destroy(g)
+
+
 def test_rk_graph_replays(gpu):
     import torch
     n = 4096
