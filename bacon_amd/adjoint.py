@@ -6,7 +6,8 @@ over shared observation times t_k, its gradient with respect to the
 per-instance parameters and initial state, and the states at the observation
 times.  `RK45Loss` wraps it in a torch.autograd.Function (used for gradient
 checks): its forward runs the forward and backward GPU passes, its backward
-scales the stored gradients by the incoming cotangent.
+scales the stored gradients by the incoming cotangent.  A `Schedule` carries
+the longest-first order from one solve to the next in a loop.
 """
 from __future__ import annotations
 
@@ -21,14 +22,35 @@ def _ptr(t):
     return None if t is None else t.data_ptr()
 
 
+class Schedule:
+    """The forward pass's schedule carried from one solve of an ensemble to
+    the next, as in an optimisation loop: after each solve it holds the
+    longest-first order by that solve's attempt counts (`longest_first`),
+    and the next solve of the same n instances runs in that order.  Results
+    do not depend on the order."""
+
+    def __init__(self):
+        self.order = None
+
+    def current(self, n: int):
+        return self.order if self.order is not None and self.order.numel() == n else None
+
+    def update(self, counts: torch.Tensor, stream=None):
+        self.order = longest_first(counts, stream)
+
+
 def solve_vjp(rhs: str, y0: torch.Tensor, params: torch.Tensor, obs_t: torch.Tensor,
               targets: torch.Tensor, t_end, tol, dt_min: float = 1e-12, dt_max: float = 0.1,
               t0: float = 0.0, checkpoint_every: int = 16, max_checkpoints: int = 128,
-              stream=None, order=None, blocks_per_cu: int = 0):
+              stream=None, order=None, blocks_per_cu: int = 0, schedule: Schedule = None):
     """All tensors float64 on the GPU: y0 [D, n], params [P, n], obs_t [n_obs],
     targets [n_obs, D, n]; t_end / tol scalars or [n] tensors.  `order` (int32
     [n], optional) is the forward pass's schedule, e.g. from
-    bg_rk_schedule_device with a plan tuned on the same problem."""
+    bg_rk_schedule_device with a plan tuned on the same problem.  With a
+    `Schedule` instead, the solve uses the order it carries (if any) and
+    leaves it the order for the next solve."""
+    if schedule is not None and order is None:
+        order = schedule.current(y0.shape[1])
     dev = y0.device
     D, n = y0.shape
     P = params.shape[0]
@@ -61,6 +83,8 @@ def solve_vjp(rhs: str, y0: torch.Tensor, params: torch.Tensor, obs_t: torch.Ten
     rc = L.lib().bg_rk_vjp_device(L.RK45, L.RHS[rhs], C.byref(d), sh)
     if rc != 0:
         raise RuntimeError(f"bg_rk_vjp_device returned {rc}")
+    if schedule is not None:
+        schedule.update(out["counts"], stream)
     return out
 
 
@@ -85,12 +109,13 @@ class RK45Loss(torch.autograd.Function):
     """loss[i] = sum_k |y_i(t_k) - target_ik|^2 with gradients w.r.t. params and y0."""
 
     @staticmethod
-    def forward(ctx, params, y0, rhs, obs_t, targets, t_end, tol):
-        r = solve_vjp(rhs, y0.detach(), params.detach(), obs_t, targets, t_end, tol)
+    def forward(ctx, params, y0, rhs, obs_t, targets, t_end, tol, schedule=None):
+        r = solve_vjp(rhs, y0.detach(), params.detach(), obs_t, targets, t_end, tol,
+                      schedule=schedule)
         ctx.save_for_backward(r["grad_params"], r["grad_y0"])
         return r["loss"]
 
     @staticmethod
     def backward(ctx, g):
         gp, gy = ctx.saved_tensors
-        return gp * g[None, :], gy * g[None, :], None, None, None, None, None
+        return gp * g[None, :], gy * g[None, :], None, None, None, None, None, None

@@ -178,3 +178,28 @@ def test_adjoint_in_measured_order_bit_identical(gpu, lv):
     g = _gpu(lv, order=order)
     for key in ("grad_params", "grad_y0", "loss", "obs_y", "counts", "status"):
         assert np.array_equal(g[key], base[key]), key
+
+
+def test_schedule_carried_across_solves(gpu, lv):
+    """A Schedule takes the longest-first order from each solve's counts and
+    hands it to the next solve, through solve_vjp and through RK45Loss; the
+    results stay bit-identical to unscheduled solves."""
+    import torch
+    from bacon_amd.adjoint import RK45Loss, Schedule
+    base = _gpu(lv)
+    sch = Schedule()
+    first = _gpu(lv, schedule=sch)
+    assert sch.order is not None and sch.order.numel() == 8
+    att = first["counts"].astype(np.int64).sum(1)
+    assert np.array_equal(sch.order.cpu().numpy(), np.argsort(-att, kind="stable"))
+    second = _gpu(lv, schedule=sch)
+    for key in ("grad_params", "grad_y0", "loss", "obs_y", "counts"):
+        assert np.array_equal(first[key], base[key]), key
+        assert np.array_equal(second[key], base[key]), key
+    params = _t(lv["params"]).requires_grad_(True)
+    y0 = _t(lv["y0"]).requires_grad_(True)
+    loss = RK45Loss.apply(params, y0, "lotka_volterra", _t(lv["obs"]), _t(lv["targets"]),
+                          15.0, TOL, sch)
+    loss.sum().backward()
+    assert np.array_equal(params.grad.cpu().numpy(), base["grad_params"])
+    assert np.array_equal(y0.grad.cpu().numpy(), base["grad_y0"])
