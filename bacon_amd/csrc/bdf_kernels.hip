@@ -76,6 +76,16 @@ __device__ __forceinline__ double xor32_f64(double v) {
     return __builtin_bit_cast(double, ((uint64_t)h << 32) | l);
 }
 
+// lane l's value, broadcast (l wave-uniform)
+__device__ __forceinline__ double readlane_lu(double v, int l) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ float readlane_lu(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
 __device__ __forceinline__ double xor32_lu(double v) { return xor32_f64(v); }
 __device__ __forceinline__ float xor32_lu(float v) { return xor32_f32(v); }
 
@@ -288,11 +298,12 @@ struct BdfLane {
     int h = 0;    // half of the wave (SPLIT = 2)
     double *lds;  // this group's W x (W+1) transpose buffer, then 2 scalars
     double *scr = nullptr;  // SPLIT = 2: this wave's kBdfScratchDoubles
-    float *prow_f = nullptr;  // SPLIT = 2, FP32 inverse: pivot row and 1 / pivot
+    float *prow_f = nullptr;  // SPLIT = 2, FP32 inverse: the pivot row
 
-    // The elimination's pivot row (W entries) and 1 / pivot (entry W), in
-    // the inverse's arithmetic type: FP64 uses the group's LDS buffer (its
-    // row 0, free during the elimination), FP32 a float buffer of its own.
+    // The elimination's pivot row (W entries) in the inverse's arithmetic
+    // type: FP64 uses the group's LDS buffer (its row 0, free during the
+    // elimination), FP32 a float buffer of its own.  1 / pivot goes from the
+    // pivot lane by readlane.
     __device__ __forceinline__ LuT *pivot_row() const {
         if constexpr (sizeof(LuT) == sizeof(double)) return lds;
         else return prow_f;
@@ -427,20 +438,23 @@ struct BdfLane {
             const LuT agk = (h == hk) ? mk : other;
             const uint32_t key = grp.max_u32(pivot_key32(agk, !used, grp.g));
             if (!pivot_nonzero32(key)) ok = false;
-            const bool piv = (pivot_lane32(key) == grp.g);
+            // every lane holds the same key (both halves reduce the same
+            // column), so the pivot row's lane is wave-uniform
+            const int pl = pivot_lane32(__builtin_amdgcn_readfirstlane(key));
+            const bool piv = (pl == grp.g);
             if (piv) {
                 used = true;
                 my_col = k;
                 my_inv = (LuT)1 / agk;
 #pragma unroll
                 for (int j = 0; j < WJ; ++j) prow[h * WJ + j] = b[j];
-                if (h == 0) prow[W] = my_inv;
                 // column k of the inverse: pivot * (1 / pivot)
                 if (h == hk) b[kk] = (LuT)1;
             }
             __builtin_amdgcn_wave_barrier();
             if (!piv) {
-                const LuT f = agk * prow[W];
+                // 1 / pivot straight from the pivot lane (half 0), not via LDS
+                const LuT f = agk * readlane_lu(my_inv, pl);
 #pragma unroll
                 for (int j = 0; j < WJ; ++j) b[j] = fma(-f, prow[h * WJ + j], b[j]);
                 // column k of the inverse: -a(g, k) / pivot
@@ -744,7 +758,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
     // one wave per block; SPLIT = 2 holds one instance per wave
     if constexpr (SPLIT == 2) L.scr = scratch + (uint64_t)blockIdx.x * kBdfScratchDoubles;
     if constexpr (SPLIT == 2 && sizeof(LuT) == sizeof(float)) {
-        __shared__ float prow_all[W + 2];
+        __shared__ float prow_all[W];
         L.prow_f = prow_all;
     }
     const uint64_t max_att = dv->max_attempts;
