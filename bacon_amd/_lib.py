@@ -9,9 +9,14 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libbacon_gpu.so")
+# BACON_GPU_LIB may name another build of the same sources (the host-side
+# sanitizer build, libbacon_gpu_san.so); there is no other implementation.
+LIB_PATH = os.environ.get("BACON_GPU_LIB") or os.path.join(HERE, "libbacon_gpu.so")
 
 OK = 0
+MINIMUM_TIME_DELTA_EXCEEDED = 8
+MAXIMUM_ITERATIONS_EXCEEDED = 9
+SINGULAR_MATRIX = 10
 DONE = 100
 ERR_INVALID_ARGUMENT = -1
 ERR_HIP = -2

@@ -61,6 +61,40 @@ def build_cpp_tests(verbose: bool = True) -> str:
     return CPP_TEST_BIN
 
 
+SAN_LIB = os.path.join(HERE, "libbacon_gpu_san.so")
+# Host-side AddressSanitizer + UndefinedBehaviorSanitizer for the C ABI layer
+# (builders, path walking, ensemble copies in api.hip); device code is built
+# as usual.  Each -fsanitize= applies to the host pass only.
+SAN_HOST_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+                  "-Xarch_host", "-fno-sanitize-recover=undefined",
+                  "-Xarch_host", "-fno-omit-frame-pointer", "-g"]
+
+
+def build_sanitized(verbose: bool = True) -> str:
+    """libbacon_gpu_san.so: api.hip's host code under ASan/UBSan, linked with
+    the other translation units of the regular build (run build() first).
+    Load it with BACON_GPU_LIB and the sanitizer runtime preloaded."""
+    os.makedirs(OBJ, exist_ok=True)
+    api_obj = os.path.join(OBJ, "api_san.o")
+    cmd = [hipcc(), *HIPFLAGS, *SAN_HOST_FLAGS, "-c", os.path.join(CSRC, "api.hip"), "-o", api_obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for api.hip (sanitized):\n{r.stdout}\n{r.stderr}")
+    objs = [api_obj] + [os.path.join(OBJ, os.path.splitext(s)[0] + ".o")
+                        for s in SOURCES if s != "api.hip"]
+    for o in objs[1:]:
+        if not os.path.exists(o):
+            raise RuntimeError(f"{o} missing: run build() first")
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-shared-libasan",
+           "-fsanitize=address,undefined", "-o", SAN_LIB, *objs]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed (sanitized):\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {SAN_LIB}")
+    return SAN_LIB
+
+
 def build(verbose: bool = True) -> str:
     os.makedirs(OBJ, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:

@@ -15,7 +15,10 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# The checker build (portable -O2); BACON_ORACLE_LIB may name another build of
+# the same source (the sanitizer build, liboracle_san.so).
+CHECKER_PATH = os.path.join(HERE, "liboracle.so")
+LIB_PATH = os.environ.get("BACON_ORACLE_LIB") or CHECKER_PATH
 
 # status codes (IVPError declaration order, src/ivp.rs:50-76)
 OK = 0
@@ -78,11 +81,22 @@ class Builder(C.Structure):
         (n, C.c_double) for n in ("tol", "dt_max", "dt_min", "t0", "t_end")]
 
 
-def build(force: bool = False) -> str:
-    """Compile liboracle.so with the committed Makefile."""
-    if force or not os.path.exists(LIB_PATH):
-        subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "liboracle.so"],
-                       check=True)
+def build(force: bool = False, target: str = "liboracle.so") -> str:
+    """Compile a Makefile target (the checker liboracle.so by default; the
+    CPU baseline's liboracle_native.so; the sanitizer build liboracle_san.so)."""
+    path = os.path.join(HERE, target)
+    if force or not os.path.exists(path):
+        cmd = ["make", "-s", "-C", HERE] + (["-B"] if force else []) + [target]
+        subprocess.run(cmd, check=True)
+    return path
+
+
+def use_native_build() -> str:
+    """Switch this process to the -O3 -march=native build, compiled here for
+    this host (the CPU baseline's timing); call before the first solve."""
+    global LIB_PATH, _lib
+    LIB_PATH = build(force=True, target="liboracle_native.so")
+    _lib = None
     return LIB_PATH
 
 
@@ -92,7 +106,7 @@ _lib = None
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        if not os.path.exists(LIB_PATH) and LIB_PATH == CHECKER_PATH:
             build()
         L = C.CDLL(LIB_PATH)
         L.or_solve.argtypes = [C.POINTER(Problem), _dp, C.POINTER(Result),
@@ -107,11 +121,14 @@ def lib():
         L.or_rhs_eval.restype = C.c_int32
         L.or_inverse.argtypes = [C.c_int32, _dp, _dp, C.POINTER(C.c_int32)]
         L.or_lu_inverse.argtypes = [C.c_int32, _dp, _dp]
+        L.or_inverse_branch.argtypes = [C.c_int32, _dp, _dp, C.c_int32]
         L.or_rk_tableau.argtypes = [C.c_int32, C.c_int32, _dp, _dp, _dp, _dp]
         L.or_bdf_coefficients.argtypes = [C.c_int32, _dp, _dp]
         L.or_adams_coefficients.argtypes = [C.c_int32, _dp, _dp, _dp]
         L.or_rootn.argtypes = [C.c_double, C.c_int32]
         L.or_rootn.restype = C.c_double
+        L.or_root8.argtypes = [C.c_double]
+        L.or_root8.restype = C.c_double
         L.or_rk_replay.argtypes = [C.POINTER(Problem), C.c_int64, _dp, _dp, _dp]
         for name in ("or_builder_new", "or_builder_new_dyn"):
             getattr(L, name).argtypes = [C.POINTER(Builder), C.c_int32, C.c_int32]
@@ -228,6 +245,20 @@ def inverse(a):
     return rc, np.asarray(inv), which.value
 
 
+INVERSE_LU, INVERSE_FULL_PIVOT, INVERSE_QR = 0, 1, 2
+
+
+def inverse_branch(a, branch):
+    """One member of the cascade alone: INVERSE_LU, INVERSE_FULL_PIVOT or
+    INVERSE_QR.  Returns (rc, inverse)."""
+    a = np.asarray(a, dtype=np.float64)
+    n = a.shape[0]
+    acm = np.asfortranarray(a)
+    inv = np.zeros((n, n), order="F")
+    rc = lib().or_inverse_branch(n, acm.ctypes.data_as(_dp), inv.ctypes.data_as(_dp), branch)
+    return rc, np.asarray(inv)
+
+
 def rk_tableau(method=RK45, semantics=INTENDED):
     c = np.zeros(6); m = np.zeros(36); b = np.zeros(6); e = np.zeros(6)
     s = lib().or_rk_tableau(method, semantics, _ptr(c), _ptr(m), _ptr(b), _ptr(e))
@@ -248,6 +279,11 @@ def adams_coefficients(method=ADAMS5):
 
 def rootn(x, n):
     return lib().or_rootn(float(x), n)
+
+
+def root8(r):
+    """The DEVICE profile's step-factor root r^(1/8), before the tangent bound."""
+    return lib().or_root8(float(r))
 
 
 def rk_replay(rhs, y0, t_seq, dt_seq, params=None, method=RK45,

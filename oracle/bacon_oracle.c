@@ -82,7 +82,7 @@ static double dev_sumsq(int n, const double *x) {
 /* r^(1/8) for r in (1e-8, 65536), built from exact operations so that CPU and
  * GPU agree bit for bit: r = m 2^e with m in [1,2) (frexp), m^(1/8) from a
  * degree-5 polynomial in (m - 1.5) (Chebyshev-node interpolant, relative
- * error < 1.2e-6), 2^(e/8) = 2^floor(e/8) * 2^((e mod 8)/8) with the second
+ * error below 1.11e-6 over [1, 2)), 2^(e/8) = 2^floor(e/8) * 2^((e mod 8)/8) with the second
  * factor a product of 2^(1/8), 2^(1/4), 2^(1/2) selected by the bits of
  * e mod 8.  A NaN argument returns NaN.  The caller bounds the result by the
  * tangent at r = 1, which decides the controller near r = 1. */
@@ -364,6 +364,7 @@ static int rk_step(rk_solver *s, double *used_dt) {
         at_stop = 1;
     }
     int clamped = 0;
+    const double dt_proposed = s->dt;
     if (s->time + s->dt >= target) {
         s->dt = target - s->time;
         clamped = 1;
@@ -424,7 +425,7 @@ static int rk_step(rk_solver *s, double *used_dt) {
         const double f0 = r <= 1e-8 ? s->one_tenth : (r >= 65536.0 ? s->four : dev_root8(r));
         /* Near r = 1 the controller's decisions hinge on the sign of
          * delta - 1 = (r - 1)/8 + O((r-1)^2), far below the polynomial's
-         * 2.4e-8 error.  r^(1/8) is concave, so its tangent at 1,
+         * 1.1e-6 relative error.  r^(1/8) is concave, so its tangent at 1,
          * 1 + (r - 1)/8, bounds it from above and is exact to second order
          * there: take the smaller of the two.  Then keep the exact power's
          * side of 1 (delta < 1 on a rejection, >= 1 on an acceptance) against
@@ -469,6 +470,11 @@ static int rk_step(rk_solver *s, double *used_dt) {
     /* rk.rs:400-412 */
     s->dt *= factor;
     if (s->dt > s->dt_max) s->dt = s->dt_max;
+    /* Extension for stops: an accepted step cut short to land on a stop hands
+     * the next attempt at least the step size proposed before the cut, so a
+     * stop just past an accepted time cannot push dt below dt_min (without
+     * stops the clamp only ever lands on the end, where the solve is done). */
+    if (accepted && clamped && at_stop && s->dt < dt_proposed) s->dt = dt_proposed;
 
     /* rk.rs:414-416 (Q8: an accepted step can still fail here) */
     if (s->dt < s->dt_min && s->time < s->end) return OR_MINIMUM_TIME_DELTA_EXCEEDED;
@@ -774,6 +780,21 @@ static int qr_inverse(int n, const double *a, double *inv) {
     free(v);
     free(beta);
     return ok;
+}
+
+/* One member of the cascade on its own (for the tests that pin each against
+ * LAPACK): branch 0 = partial-pivot LU, 1 = full-pivot LU, 2 = QR. */
+int32_t or_inverse_branch(int32_t n, const double *a, double *inv, int32_t branch) {
+    switch (branch) {
+    case 0:
+        return or_lu_inverse(n, a, inv);
+    case 1:
+        return full_piv_inverse(n, a, inv) ? OR_OK : OR_SINGULAR_MATRIX;
+    case 2:
+        return qr_inverse(n, a, inv) ? OR_OK : OR_SINGULAR_MATRIX;
+    default:
+        return OR_INVALID_ARGUMENT;
+    }
 }
 
 int32_t or_inverse(int32_t n, const double *a, double *inv, int32_t *which) {
@@ -1407,6 +1428,7 @@ int32_t or_adams_coefficients(int32_t method, double *pred, double *corr, double
 }
 
 double or_rootn(double x, int32_t n) { return dev_rootn(x, n); }
+double or_root8(double r) { return dev_root8(r); }
 
 static int adams_solve(const or_problem *p, double *y_out, or_result *r, or_path *path) {
     adams_solver *s = (adams_solver *)malloc(sizeof(adams_solver));

@@ -17,10 +17,15 @@
  *                       src/ivp/rk.rs:400-401.
  *   OR_PROFILE_DEVICE : the same algorithm with the operation order of the
  *                       gfx950 kernels: explicit fma() at each multiply-add,
- *                       zero tableau entries skipped, and x^(1/4) computed as
- *                       sqrt(sqrt(x)).  Every operation is an IEEE correctly
- *                       rounded primitive on both CPU and GPU, so the
- *                       single-lane GPU RK kernels must match it bit for bit.
+ *                       zero tableau entries skipped, and the controller on
+ *                       squared norms (accept ss <= (tol dt)^2; step factor
+ *                       r^(1/8) with r = (tol dt)^2 / ss from or_root8's
+ *                       frexp / degree-5 polynomial / ldexp, relative error
+ *                       below 1.11e-6, bounded by the tangent at r = 1).
+ *                       Every operation is an IEEE correctly rounded
+ *                       primitive or an exact bit operation on both CPU and
+ *                       GPU, so the single-lane GPU RK kernels must match it
+ *                       bit for bit.
  *
  * Pinning (tests/test_oracle.py): the reference's own ivp tests, order
  * conditions, closed forms, scipy golden vectors, LAPACK inverses.
@@ -159,6 +164,8 @@ int32_t or_builder_solve(const or_builder *b);
  * a and inv are column-major n*n.  *which receives 0/1/2 for the method used. */
 int32_t or_inverse(int32_t n, const double *a, double *inv, int32_t *which);
 int32_t or_lu_inverse(int32_t n, const double *a, double *inv);
+/* one member of the cascade: branch 0 = LU, 1 = full-pivot LU, 2 = QR */
+int32_t or_inverse_branch(int32_t n, const double *a, double *inv, int32_t branch);
 
 /* Tableau exposure for order-condition tests: semantics picks the stage
  * matrix orientation (Q1).  Arrays: c[s], m[s*s] row-major as used by the
@@ -171,6 +178,8 @@ int32_t or_bdf_coefficients(int32_t method, double *higher, double *lower);
 int32_t or_adams_coefficients(int32_t method, double *pred, double *corr, double *err);
 /* x^(1/n), n = 3 or 5, the DEVICE profile's exact-operation root */
 double or_rootn(double x, int32_t n);
+/* the DEVICE profile's step-factor root r^(1/8) (before the tangent bound) */
+double or_root8(double r);
 
 /* Discrete adjoint of the accepted RK45 (intended) step sequence and a frozen
  * step replay, used to check the GPU vjp.  See oracle/bacon_oracle.c. */

@@ -11,7 +11,8 @@ transposed matrix, a dropped term, a wrong sign or index) fails it:
   * independent high-accuracy scipy solutions (tests/golden/*.npz), an exact
     matrix-exponential solution for the linear workload, and the conserved
     quantity of Lotka-Volterra;
-  * LAPACK inverses for the nalgebra LU restatement.
+  * LAPACK inverses for each member of the nalgebra inverse cascade (LU,
+    full-pivot LU, QR) and constructed inputs that fix which member answers.
 """
 from fractions import Fraction
 
@@ -289,6 +290,90 @@ def test_inverse_needs_pivoting(orc):
     assert rc == O.OK and np.array_equal(inv, a)
 
 
+def _pivot_heavy(rng, n):
+    """A random matrix whose largest entries sit off the diagonal in shuffled
+    rows and columns, so row and column pivots both move at every stage."""
+    a = rng.standard_normal((n, n))
+    a[rng.permutation(n), rng.permutation(n)] += 8.0 * np.sign(rng.standard_normal(n))
+    return a
+
+
+@pytest.mark.parametrize("branch", [O.INVERSE_LU, O.INVERSE_FULL_PIVOT, O.INVERSE_QR])
+def test_each_inverse_matches_lapack(orc, branch):
+    """Every member of the cascade (bdf.rs:429-441: lu, full_piv_lu, qr) on
+    its own against LAPACK, at every size 1..32, on plain Gaussian matrices
+    and on matrices that force row and column interchanges.  A wrong sign in
+    a reflection, a permutation applied in the wrong order or to the wrong
+    side, or a dropped update fails this."""
+    rng = np.random.default_rng(11 + branch)
+    for n in range(1, 33):
+        for a in (rng.standard_normal((n, n)), _pivot_heavy(rng, n)):
+            rc, inv = O.inverse_branch(a, branch)
+            assert rc == O.OK, (branch, n)
+            ref = np.linalg.inv(a)
+            scale = np.linalg.cond(a) * max(1.0, np.abs(ref).max())
+            assert np.abs(inv - ref).max() <= 1e-13 * n * scale, (branch, n)
+
+
+def _residual_ratio(a, x):
+    """max |A X - I| / (|A| |X|): small for a backward-stable inverse whatever
+    the conditioning."""
+    return np.abs(a @ x - np.eye(a.shape[0])).max() / (
+        np.abs(a).sum(1).max() * np.abs(x).sum(0).max())
+
+
+def test_inverse_cascade_branches(orc):
+    """Which member of the cascade answers (bdf.rs:429-444).  The inputs are
+    built so that plain double arithmetic, restated here independently of the
+    oracle, fixes the outcome of each factorisation."""
+    # 1. well conditioned: partial-pivot LU answers
+    a = np.array([[4.0, 1.0, 0.5], [1.0, 3.0, 0.25], [0.5, 0.25, 2.0]])
+    rc, inv, which = O.inverse(a)
+    assert rc == O.OK and which == 0
+    assert np.abs(inv - np.linalg.inv(a)).max() < 1e-14
+
+    # 2. partial-pivot LU meets an exact zero pivot, full-pivot LU does not.
+    #    [[p, b], [c, d]] with |p| > |c|: LU pivots on p, the multiplier is
+    #    l = c * (1/p), and the trailing entry -b*l + d is exactly 0 when
+    #    d = fl(b*l).  Full pivoting takes b (the largest entry) first; after
+    #    the column interchange the trailing entry is -p*l2 + c with
+    #    l2 = d * (1/b), which is not zero.
+    p, b, c = 26.0, 48.0, 23.0
+    l = c * (1.0 / p)
+    d = b * l
+    assert -b * l + d == 0.0
+    l2 = d * (1.0 / b)
+    assert -p * l2 + c != 0.0
+    a = np.array([[p, b], [c, d]])
+    assert O.inverse_branch(a, O.INVERSE_LU)[0] == O.SINGULAR_MATRIX
+    rc_f, inv_f = O.inverse_branch(a, O.INVERSE_FULL_PIVOT)
+    rc, inv, which = O.inverse(a)
+    assert rc == O.OK and which == 1 and rc_f == O.OK
+    assert np.array_equal(inv, inv_f) and np.all(np.isfinite(inv))
+    assert _residual_ratio(a, inv) < 1e-14
+
+    # 3. rank one, both LU forms meet an exact zero pivot, Householder QR
+    #    leaves a rounding residue on R's diagonal and answers (nalgebra's
+    #    QR try_inverse only rejects an exact zero diagonal).
+    a = np.array([[1.0, 2.0], [2.0, 4.0]])
+    assert -4.0 * (1.0 * 0.5) + 2.0 == 0.0  # LU: pivot 2, l = 1*(1/2)
+    assert -2.0 * (2.0 * 0.25) + 1.0 == 0.0  # full pivot: 4 first, l = 2*(1/4)
+    assert O.inverse_branch(a, O.INVERSE_LU)[0] == O.SINGULAR_MATRIX
+    assert O.inverse_branch(a, O.INVERSE_FULL_PIVOT)[0] == O.SINGULAR_MATRIX
+    rc, inv, which = O.inverse(a)
+    assert rc == O.OK and which == 2
+    assert np.array_equal(inv, O.inverse_branch(a, O.INVERSE_QR)[1])
+
+    # 4. an exact zero column survives every elimination and reflection
+    #    exactly: all three fail and the cascade reports SingularMatrix
+    for a in (np.array([[1.0, 0.0], [2.0, 0.0]]), np.zeros((3, 3)),
+              np.array([[2.0, 0.0, 1.0], [1.0, 0.0, 3.0], [5.0, 0.0, 4.0]])):
+        for br in (O.INVERSE_LU, O.INVERSE_FULL_PIVOT, O.INVERSE_QR):
+            assert O.inverse_branch(a, br)[0] == O.SINGULAR_MATRIX
+        rc, _, which = O.inverse(a)
+        assert rc == O.SINGULAR_MATRIX and which == -1
+
+
 # ---------------------------------------------------------------- golden solutions
 def test_lorenz_vs_dop853(orc):
     g = np.load("tests/golden/lorenz_golden.npz")
@@ -356,8 +441,9 @@ def test_brusselator_bdf6_vs_radau(orc):
 
 # ---------------------------------------------------------------- profiles
 def test_device_profile_agrees_with_bacon_profile(orc):
-    """The fused/sqrt-sqrt arithmetic of the kernels changes results only at
-    rounding level: on the plain Lorenz workload the two profiles' final
+    """The kernels' arithmetic (fused multiply-adds, the squared-norm
+    controller with the polynomial root) changes results only at the
+    tolerance's scale: on the plain Lorenz workload the two profiles' final
     states agree within 10x the requested tolerance."""
     w = W.lorenz_plain(128)
     outs = []
@@ -441,6 +527,50 @@ def test_rootn_accuracy(orc, n):
         assert abs(got / want - 1.0) < 3e-7, (x, got, want)
     assert O.rootn(0.0, n) == 0.0 and O.rootn(np.inf, n) == np.inf
     assert np.isnan(O.rootn(np.nan, n))
+
+
+@pytest.mark.parametrize("profile", [O.PROFILE_BACON, O.PROFILE_DEVICE])
+def test_stop_next_to_accepted_step(orc, profile):
+    """A stop dt_min/8 after an accepted step time: the step cut short to land
+    on it is accepted, and the next attempt gets the step size proposed
+    before the cut, so the solve still succeeds (shrinking dt to the cut
+    would fail it with MinimumTimeDeltaExceeded).  Up to that step the path
+    is the stop-free path, bit for bit."""
+    w = W.lorenz_plain(2)
+    kw = dict(params=w["params"][:, 0], semantics=O.INTENDED, profile=profile, path_cap=200000)
+    free = O.solve(O.RK45, "lorenz", w["y0"][:, 0], 0.0, 10.0, 1e-6, 1e-6, 0.1, **kw)
+    assert free["status"] == O.OK
+    k = 100
+    stop = free["path_t"][k] + 1e-6 / 8
+    r = O.solve(O.RK45, "lorenz", w["y0"][:, 0], 0.0, 10.0, 1e-6, 1e-6, 0.1,
+                stops=np.array([stop]), **kw)
+    assert r["status"] == O.OK and r["t"] == 10.0
+    assert np.array_equal(r["path_t"][:k + 1], free["path_t"][:k + 1])
+    assert np.array_equal(r["path_y"][:k + 1], free["path_y"][:k + 1])
+    assert r["path_t"][k + 1] == stop
+    # the step after the landing is the one the controller proposed at t_k
+    # (a full step; the cut one was 1.25e-7 long)
+    assert r["path_dt"][k + 2] == free["path_dt"][k + 1]
+
+
+def test_root8_accuracy(orc):
+    """The DEVICE controller's r^(1/8) against pow: relative error below
+    1.11e-6 over the whole controller range (1e-8, 65536), a dense sweep of
+    every binade and every residue of the exponent mod 8 (the 2^(k/8)
+    scale table); exact at powers of 2^8; NaN in, NaN out."""
+    rng = np.random.default_rng(8)
+    xs = np.concatenate([np.exp(rng.uniform(np.log(1e-8), np.log(65536.0), 200000)),
+                         2.0 ** np.arange(-26, 16, dtype=float),
+                         np.nextafter(1.0, 0.0) ** np.arange(1, 4), [1.0, 1.5, 1.9999999]])
+    worst = 0.0
+    for x in xs:
+        got = O.root8(x)
+        worst = max(worst, abs(got / x ** 0.125 - 1.0))
+    assert worst < 1.11e-6, worst
+    assert worst > 1e-6  # the bound is the polynomial's, not a looser one
+    for k in range(-3, 3):
+        assert O.root8(2.0 ** (8 * k)) == 2.0 ** k * O.root8(1.0)
+    assert np.isnan(O.root8(np.nan))
 
 
 def test_adams_as_built_derivative_history_is_one_step_stale(orc):
