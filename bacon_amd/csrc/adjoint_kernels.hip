@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(256)
                 } else {
                     const bool to_stop = next_stop < n_stops;
                     bool clamped;
-                    const int r = L.attempt_to(target, clamped);
+                    const int r = L.attempt_to(target, to_stop, clamped);
                     ++attempts;
                     if (r == STEP_ACCEPTED) {
                         ++nacc;
@@ -403,7 +403,7 @@ __global__ void __launch_bounds__(256)
                         for (int c = 0; c < D; ++c) y_before[c] = L.y[c];
                         const double dt_used = (L.t + L.dt >= target) ? target - L.t : L.dt;
                         bool clamped;
-                        const int r = L.attempt_to(target, clamped);
+                        const int r = L.attempt_to(target, to_stop, clamped);
                         if (r == STEP_ACCEPTED) {
                             const bool at_stop = clamped && to_stop;
                             if (at_stop) {

@@ -219,7 +219,7 @@ struct RhsOscillator {
 // r^(1/8) for r in (1e-8, 65536) from exact operations only (frexp, fma,
 // ldexp, products of correctly rounded constants): r = m 2^e, m in [1,2);
 // m^(1/8) by a degree-5 polynomial in (m - 1.5) interpolating at Chebyshev
-// nodes (relative error < 1.2e-6); 2^(e/8) = 2^floor(e/8) times a product of
+// nodes (relative error below 1.11e-6); 2^(e/8) = 2^floor(e/8) times a product of
 // 2^(1/8), 2^(1/4), 2^(1/2) selected by the bits of e mod 8.  NaN in, NaN
 // out.  Used as the step-size factor (tol/err)^(1/4); the caller bounds it
 // by the tangent at r = 1.
@@ -287,14 +287,19 @@ struct RkLane {
     // STEP_ACCEPTED, STEP_REJECTED or an IVPError code.
     __device__ __forceinline__ int attempt() {
         bool clamped;
-        return attempt_to(end, clamped);
+        return attempt_to(end, false, clamped);
     }
 
     // The same attempt with the end clamp of rk.rs:366-368 aimed at `target`
-    // (the end, or an observation time before it); `clamped` reports whether
-    // dt was cut to reach it.
-    __device__ __forceinline__ int attempt_to(double target, bool &clamped) {
+    // (the end, or an observation time before it: `to_stop`); `clamped`
+    // reports whether dt was cut to reach it.  An accepted step cut short to
+    // land on a stop hands the next attempt at least the step size proposed
+    // before the cut: a stop just past an accepted time must not shrink dt
+    // below dt_min and fail a solve that succeeds without the stop (at the
+    // end the solve is done, so the reference clamp is unaffected).
+    __device__ __forceinline__ int attempt_to(double target, bool to_stop, bool &clamped) {
         clamped = false;
+        const double dt_proposed = dt;
         if (t + dt >= target) {
             dt = target - t;
             clamped = true;
@@ -368,6 +373,7 @@ struct RkLane {
         }
         // dt is never NaN here (factor is finite), so the clamp is a minimum
         dt = fmin(dt * factor, dt_max);
+        if (accepted && clamped && to_stop) dt = fmax(dt, dt_proposed);
         // rk.rs:414-416
         if (dt < dt_min && t < end) return BG_MINIMUM_TIME_DELTA_EXCEEDED;
         return accepted ? STEP_ACCEPTED : STEP_REJECTED;

@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(256)
                 } else {
                     const bool to_stop = next_stop < n_stops;
                     bool clamped;
-                    const int r = L.attempt_to(target, clamped);
+                    const int r = L.attempt_to(target, to_stop, clamped);
                     ++attempts;
                     if (r == STEP_ACCEPTED) {
                         ++nacc;

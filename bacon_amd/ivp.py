@@ -264,6 +264,12 @@ class Ensemble:
         return self
 
     def with_derivative(self, rhs, params_soa=None, shared=None):
+        """params_soa: [P, n] per-instance parameters (structure of arrays,
+        P * n values), or None for a right-hand side without parameters."""
+        if params_soa is not None:
+            shape = np.shape(params_soa)
+            if len(shape) != 2 or shape[1] != self.n:
+                raise ValueError(f"params_soa must have shape (P, {self.n}); got {shape}")
         p = self._arr(np.zeros(0) if params_soa is None else np.asarray(params_soa).reshape(-1))
         n_params = 0 if params_soa is None else int(np.asarray(params_soa).shape[0])
         s = self._arr(np.zeros(0) if shared is None else np.asarray(shared).reshape(-1))

@@ -76,6 +76,20 @@ BYTES_PER_INSTANCE = 8 * 8 + 4 + 8 * 4 + 4 + 8
 MEASURED_TRAFFIC_BYTES = 1.065e9 + 0.237e9
 
 
+_BASELINE_ORACLE = None
+
+
+def baseline_oracle():
+    """The CPU oracle in its baseline build (-O3 -march=native, compiled once
+    per run on this host)."""
+    global _BASELINE_ORACLE
+    if _BASELINE_ORACLE is None:
+        import oracle
+        oracle.use_native_build()
+        _BASELINE_ORACLE = oracle
+    return _BASELINE_ORACLE
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -765,7 +779,7 @@ def main():
                "hbm_traffic_bytes_rocprof": 68.4e6 + 68.6e6,
                "algorithmic_hbm_bytes": nl * LIN_N * 8 * 2 + nl * 24}
         if not args.no_cpu_baseline:
-            import oracle as O
+            O = baseline_oracle()
             m, threads = 64, min(os.cpu_count() or 1, 16)
             tcl = time.perf_counter()
             O.solve_ensemble(O.RK45, "linear", wl["y0"][:, :m], None, wl["t_end"][:m],
@@ -786,7 +800,7 @@ def main():
         result["graph_replay"] = graph_replay(cx)
 
     if not args.no_cpu_baseline:
-        import oracle as O
+        O = baseline_oracle()
         threads = min(os.cpu_count() or 1, 16)
         log(f"[bench] CPU oracle baseline ({threads} threads, ~{args.cpu_seconds}s)")
 
@@ -820,7 +834,8 @@ def main():
         result["cpu_baseline"] = {
             "value": rate, "unit": "solves/s", "cores": threads, "kind": "port",
             "sample": f"first {done} instances of the same workload, C oracle "
-                      f"(bacon arithmetic, -O2), OpenMP over {threads} threads; median of 3 "
+                      f"(bacon arithmetic, gcc -O3 -march=native on this host), OpenMP over "
+                      f"{threads} threads; median of 3 "
                       f"runs over that sample, {args.cpu_seconds:.0f} s in all",
             "runs": rates,
             "single_thread": {"value": rate1, "unit": "solves/s", "runs": rates1,

@@ -147,7 +147,9 @@ int bg_ensemble_with_initial_time(bg_ensemble *e, const double *v, uint64_t coun
 int bg_ensemble_with_ending_time(bg_ensemble *e, const double *v, uint64_t count);
 /* y0 structure-of-arrays: y0[d*n + i]; count = dim*n */
 int bg_ensemble_with_initial_conditions(bg_ensemble *e, const double *y0, uint64_t count);
-/* params structure-of-arrays: params[k*n + i]; shared: RHS-wide data */
+/* params structure-of-arrays: params[k*n + i], k < n_params, i < n, so the
+ * array holds n_params * n doubles; shared: RHS-wide data of n_shared
+ * doubles */
 int bg_ensemble_with_derivative(bg_ensemble *e, int rhs, const double *params,
                                 uint32_t n_params, const double *shared, uint64_t n_shared);
 int bg_ensemble_with_semantics(bg_ensemble *e, int semantics);

@@ -1,11 +1,15 @@
 """GPU discrete adjoint (bg_rk_vjp_device) against independent references.
 
   * forward pass with observation stops: bit-identical to the oracle's
-    DEVICE-profile solve with the same stops (states at every observation);
+    DEVICE-profile solve with the same stops (states at every observation),
+    and within 10x tol of the oracle's BACON profile (the reference's own
+    arithmetic: no fused multiply-adds, nalgebra's norm, powf) at every
+    observation of 256 instances;
   * gradients vs central finite differences (h = 1e-6) of the loss evaluated
     by replaying the frozen accepted-step sequence in the oracle: relative
-    error below 1e-6 per instance (the step sequence comes from the oracle's
-    own forward solve, which the first test shows equals the GPU's);
+    error below 1e-6 per instance on 64 instances drawn over the whole
+    parameter range (the step sequence comes from the oracle's own forward
+    solve, which the first test shows equals the GPU's);
   * gradients vs the oracle's forward-sensitivity solve of the augmented
     14-dimensional Lotka-Volterra system at tol 1e-12: the continuous and
     discrete sensitivities differ by the discretisation error, so the bar is
@@ -21,6 +25,7 @@ from bacon_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-8
+N = 64
 
 
 def _t(a, dtype=None):
@@ -31,13 +36,18 @@ def _t(a, dtype=None):
 
 @pytest.fixture(scope="module")
 def lv():
-    """8 Lotka-Volterra instances of the adjoint workload, targets = the GPU
+    """64 Lotka-Volterra instances of the adjoint workload (parameters
+    log-uniform over [0.5, 2], y0 uniform over [0.5, 2]^2), targets = the GPU
     solution at the perturbed parameters."""
     import torch
     from bacon_amd.adjoint import solve_vjp
-    w = W.lotka_volterra(8)
+    w = W.lotka_volterra(N)
+    lo, hi = np.log(0.5), np.log(2.0)
+    lp = np.log(w["params"])
+    # the sample spans the range: every parameter reaches both outer fifths
+    assert np.all(lp.min(1) < lo + 0.2 * (hi - lo)) and np.all(lp.max(1) > hi - 0.2 * (hi - lo))
     obs = w["obs"]
-    zeros = torch.zeros((obs.size, 2, 8), dtype=torch.float64, device="cuda:0")
+    zeros = torch.zeros((obs.size, 2, N), dtype=torch.float64, device="cuda:0")
     r = solve_vjp("lotka_volterra", _t(w["y0"]), _t(w["target_params"]), _t(obs), zeros,
                   15.0, TOL)
     assert torch.all(r["status"] == 0)
@@ -62,7 +72,7 @@ def _oracle_path(w, i, params=None):
 def test_forward_with_stops_bit_exact(gpu, orc, lv):
     g = _gpu(lv)
     assert np.all(g["status"] == 0)
-    for i in range(8):
+    for i in range(N):
         r = _oracle_path(lv, i)
         assert r["status"] == 0
         assert np.array_equal(g["obs_y"][:-1, :, i], r["stop_y"])
@@ -91,7 +101,7 @@ def _frozen_loss(w, i, r, params, y0):
 def test_vjp_vs_frozen_step_finite_differences(gpu, orc, lv):
     g = _gpu(lv)
     h = 1e-6
-    for i in range(8):
+    for i in range(N):
         r = _oracle_path(lv, i)
         p0, y00 = lv["params"][:, i].copy(), lv["y0"][:, i].copy()
         base = _frozen_loss(lv, i, r, p0, y00)
@@ -110,7 +120,7 @@ def test_vjp_vs_frozen_step_finite_differences(gpu, orc, lv):
 
 def test_vjp_vs_forward_sensitivity(gpu, orc, lv):
     g = _gpu(lv)
-    for i in range(8):
+    for i in range(N):
         z0 = np.zeros(14)
         z0[:2] = lv["y0"][:, i]
         z0[10] = 1.0  # dx/dx0
@@ -136,10 +146,10 @@ def test_autograd_function(gpu, lv):
     y0 = _t(lv["y0"]).requires_grad_(True)
     loss = RK45Loss.apply(params, y0, "lotka_volterra", _t(lv["obs"]), _t(lv["targets"]),
                           15.0, TOL)
-    weights = torch.arange(1, 9, dtype=torch.float64, device="cuda:0")
+    weights = torch.arange(1, N + 1, dtype=torch.float64, device="cuda:0")
     (loss * weights).sum().backward()
     g = _gpu(lv)
-    w = np.arange(1, 9)
+    w = np.arange(1, N + 1)
     assert np.allclose(params.grad.cpu().numpy(), g["grad_params"] * w, rtol=1e-14, atol=0)
     assert np.allclose(y0.grad.cpu().numpy(), g["grad_y0"] * w, rtol=1e-14, atol=0)
 
@@ -189,7 +199,7 @@ def test_schedule_carried_across_solves(gpu, lv):
     base = _gpu(lv)
     sch = Schedule()
     first = _gpu(lv, schedule=sch)
-    assert sch.order is not None and sch.order.numel() == 8
+    assert sch.order is not None and sch.order.numel() == N
     att = first["counts"].astype(np.int64).sum(1)
     assert np.array_equal(sch.order.cpu().numpy(), np.argsort(-att, kind="stable"))
     second = _gpu(lv, schedule=sch)
@@ -203,3 +213,57 @@ def test_schedule_carried_across_solves(gpu, lv):
     loss.sum().backward()
     assert np.array_equal(params.grad.cpu().numpy(), base["grad_params"])
     assert np.array_equal(y0.grad.cpu().numpy(), base["grad_y0"])
+
+
+def test_forward_vs_bacon_profile(gpu, orc):
+    """The GPU forward pass against the reference's own arithmetic (the
+    oracle's BACON profile: separate multiply and add, err = |E|/dt compared
+    with tol, delta = (tol/err)^(1/4) by pow) at all 20 observation times of
+    256 instances: within 10x tol everywhere.  Lotka-Volterra orbits are
+    periodic, so differences between two tol-controlled solves do not grow
+    exponentially over [0, 15]."""
+    import torch
+    from bacon_amd.adjoint import solve_vjp
+    n = 256
+    w = W.lotka_volterra(n)
+    obs = w["obs"]
+    zeros = torch.zeros((obs.size, 2, n), dtype=torch.float64, device="cuda:0")
+    g = solve_vjp("lotka_volterra", _t(w["y0"]), _t(w["params"]), _t(obs), zeros, 15.0, TOL)
+    assert torch.all(g["status"] == 0)
+    obs_y = g["obs_y"].cpu().numpy()
+    worst = 0.0
+    for i in range(n):
+        r = O.solve(O.RK45, "lotka_volterra", w["y0"][:, i], 0.0, 15.0, TOL, 1e-12, 0.1,
+                    params=w["params"][:, i], semantics=O.INTENDED, profile=O.PROFILE_BACON,
+                    stops=obs[:-1])
+        assert r["status"] == 0
+        ref = np.vstack([r["stop_y"], r["y"][None]])
+        worst = max(worst, np.abs(obs_y[:, :, i] - ref).max())
+    assert worst <= 10 * TOL, worst
+
+
+def test_observation_next_to_accepted_step(gpu, orc):
+    """An observation time dt_min/8 after an accepted step of the forward
+    pass: the landing keeps the controller's step size, so the solve
+    finishes (a cut-down dt would fall below dt_min), the forward states are
+    bit-identical to the oracle's, and the gradient is finite."""
+    import torch
+    from bacon_amd.adjoint import solve_vjp
+    w = W.lotka_volterra(1)
+    dt_min = 1e-12
+    free = O.solve(O.RK45, "lotka_volterra", w["y0"][:, 0], 0.0, 15.0, TOL, dt_min, 0.1,
+                   params=w["params"][:, 0], semantics=O.INTENDED, profile=O.PROFILE_DEVICE,
+                   path_cap=100000)
+    obs = np.array([free["path_t"][40] + dt_min / 8, 7.5, 15.0])
+    zeros = torch.zeros((obs.size, 2, 1), dtype=torch.float64, device="cuda:0")
+    g = solve_vjp("lotka_volterra", _t(w["y0"]), _t(w["params"]), _t(obs), zeros, 15.0, TOL,
+                  dt_min=dt_min)
+    assert int(g["status"][0]) == 0
+    r = O.solve(O.RK45, "lotka_volterra", w["y0"][:, 0], 0.0, 15.0, TOL, dt_min, 0.1,
+                params=w["params"][:, 0], semantics=O.INTENDED, profile=O.PROFILE_DEVICE,
+                stops=obs[:-1])
+    assert r["status"] == 0
+    obs_y = g["obs_y"].cpu().numpy()
+    assert np.array_equal(obs_y[:-1, :, 0], r["stop_y"])
+    assert np.array_equal(obs_y[-1, :, 0], r["y"])
+    assert torch.all(torch.isfinite(g["grad_params"])) and torch.all(torch.isfinite(g["grad_y0"]))

@@ -48,6 +48,24 @@ def test_linear_vs_oracle_and_closed_form(gpu, orc):
         assert abs(int(g["n_accepted"][i]) - r["n_accepted"]) <= 0.02 * r["n_accepted"] + 1
 
 
+def test_linear_vs_bacon_profile(gpu, orc):
+    """64 instances of the shared-matrix workload (tol 1e-9 on [0, 5])
+    against the reference's own arithmetic (the oracle's BACON profile:
+    separate multiply and add in the stage sums and in A y, the norm-based
+    controller with pow): within 10x tol everywhere."""
+    n = 64
+    w = W.linear_shared(n)
+    g = ivp.solve_ensemble(L.RK45, w)
+    assert np.all(g["status"] == 0)
+    worst = 0.0
+    for i in range(n):
+        r = O.solve(O.RK45, "linear", w["y0"][:, i], 0.0, 5.0, 1e-9, 1e-12, 0.1,
+                    shared=w["shared"], semantics=O.INTENDED, profile=O.PROFILE_BACON)
+        assert r["status"] == 0
+        worst = max(worst, np.abs(g["y"][:, i] - r["y"]).max())
+    assert worst <= 10 * 1e-9, worst
+
+
 def test_linear_per_instance_ends_and_tolerances(gpu, orc):
     """Per-instance t_end in {1, 2.5, 5} and tol in {1e-6, 1e-8, 1e-9}, 24
     instances, mixed in one tile: each meets 2 tol max(1, |y|) against the

@@ -91,6 +91,25 @@ def test_every_kth_and_ring_wrap(gpu, orc, w):
             assert np.array_equal(h["path_y"][j, :, i], r["path_y"][last])
 
 
+def test_stop_next_to_accepted_step(gpu, orc, w):
+    """A stop dt_min/8 after instance 0's 100th accepted time: the landing
+    keeps the controller's step size and every instance finishes, bit for bit
+    with the oracle's path."""
+    k = 100
+    stop = _oracle(w, 0)["path_t"][k] + 1e-6 / 8
+    stops = np.array([stop, 5.0])
+    g = ivp.solve_path(L.RK45, w, capacity=4096, stops=stops)
+    assert np.all(g["status"] == 0)
+    for i in (0, 1, 2, 50):
+        r = _oracle(w, i, stops=stops)
+        assert r["status"] == 0
+        m = r["n_accepted"]
+        assert g["path_len"][i] == m
+        assert np.array_equal(g["path_t"][:m, i], r["path_t"])
+        assert np.array_equal(g["path_y"][:m, :, i], r["path_y"])
+    assert g["path_t"][k + 1, 0] == stop
+
+
 def test_invalid_arguments(gpu, w):
     import ctypes as C
     d = L.EnsembleDesc(n=4, dim=3, n_params=3)
