@@ -40,7 +40,13 @@ def test_cpu_suite_under_sanitizers():
     cmd = [sys.executable, "-m", "pytest", os.path.join(ROOT, "tests"), "-m", "not gpu", "-q",
            "-x", "-p", "no:cacheprovider",
            "--ignore", os.path.join(ROOT, "tests", "test_sanitizers.py")]
-    r = subprocess.run(cmd, env=env, cwd=ROOT, capture_output=True, text=True, timeout=900)
+    try:
+        r = subprocess.run(cmd, env=env, cwd=ROOT, capture_output=True, text=True, timeout=900)
+    finally:
+        # rebuilt on every run; not left in the tree that ships to GPU runs
+        for p in (san_oracle, san_gpu):
+            if os.path.exists(p):
+                os.remove(p)
     out = r.stdout + r.stderr
     assert "AddressSanitizer" not in out and "runtime error" not in out, out[-4000:]
     assert r.returncode == 0, out[-4000:]
