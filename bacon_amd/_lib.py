@@ -140,6 +140,8 @@ SIGNATURES = [
     ("bg_ensemble_destroy", None, [C.c_void_p]),
     ("bg_rk_solve_device", C.c_int,
      [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.POINTER(Launch), C.c_void_p]),
+    ("bg_rk_solve_device_noalloc", C.c_int,
+     [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.POINTER(Launch), C.c_void_p]),
     ("bg_rk_solve_path_device", C.c_int,
      [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.POINTER(PathDesc), C.c_void_p]),
     ("bg_rk_autotune_device", C.c_int,

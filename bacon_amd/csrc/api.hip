@@ -754,6 +754,49 @@ extern "C" int bg_rk_solve_device(int method, int rhs, int semantics, const bg_e
     return rk_launch_ensemble(method, rhs, semantics, *d, lc, (hipStream_t)stream);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free RK launch.  The persistent launch's workspace (the work
+// counter at [0, 256), the descriptor copy after it) lives in this file-scope
+// device array instead of coming from the stream-ordered allocator.  Each
+// GPU holds its own copy until the process ends; its address is looked up
+// once per device (hipGetSymbolAddress, with that device current) and only
+// that address is handed to kernels.  The setup kernel rewrites the counter
+// and the descriptor at the start of every launch, so nothing carries over
+// from the previous call; calls that overlap on one GPU share the array and
+// race.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = 1024;
+static_assert(kLaunchWorkspaceBytes <= kNoAllocScratchBytes,
+              "the RK launch workspace must fit the allocation-free scratch");
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static void *g_scratch_addr[64];
+
+// scratch's device address on the current device (nullptr on failure)
+static void *noalloc_scratch(int dev) {
+    std::lock_guard<std::mutex> lk(g_dev_mutex);
+    if (!g_scratch_addr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+        g_scratch_addr[dev] = p;
+    }
+    return g_scratch_addr[dev];
+}
+
+extern "C" int bg_rk_solve_device_noalloc(int method, int rhs, int semantics,
+                                          const bg_ensemble_desc *d, const bg_launch *launch,
+                                          void *stream) {
+    if (!d || !is_rk(method)) return BG_ERR_INVALID_ARGUMENT;
+    int dev = 0;
+    BG_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return BG_ERR_UNSUPPORTED;
+    void *ws = noalloc_scratch(dev);
+    if (!ws) return BG_ERR_HIP;
+    bg_launch lc;
+    memset(&lc, 0, sizeof lc);
+    if (launch) lc = *launch;
+    return rk_launch_ensemble(method, rhs, semantics, *d, lc, (hipStream_t)stream, ws);
+}
+
 extern "C" int bg_rk_solve_path_device(int method, int rhs, int semantics,
                                        const bg_ensemble_desc *d, const bg_path_desc *path,
                                        void *stream) {

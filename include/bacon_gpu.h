@@ -213,6 +213,15 @@ typedef struct {
  * (hipStream_t, NULL = default stream).  Asynchronous. */
 int bg_rk_solve_device(int method, int rhs, int semantics, const bg_ensemble_desc *d,
                        const bg_launch *launch, void *stream);
+/* The same launch without any allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * launch's workspace (work counter and descriptor copy, 1 KiB at most: it
+ * does not grow with d->n) is a static device array of the library, one per
+ * GPU, reused by every call on that GPU: calls that overlap on one GPU race.
+ * Same arguments, results and errors as bg_rk_solve_device.  Asynchronous. */
+int bg_rk_solve_device_noalloc(int method, int rhs, int semantics, const bg_ensemble_desc *d,
+                               const bg_launch *launch, void *stream);
 /* The plain per-instance path over the same descriptor (synchronous). */
 int bg_rk_solve_plain_device(int method, int rhs, int semantics, const bg_ensemble_desc *d,
                              void *stream);
