@@ -240,15 +240,27 @@ def time_launches(cx, fn, steps, warmup):
 
 
 def cpu_model():
-    """The host CPU's model name, from /proc/cpuinfo."""
+    """The host CPU: /proc/cpuinfo's model name, family, model and stepping
+    (a virtualised host may report only the vendor's family name), and the
+    CPU flags that decide the -march=native build's vector width."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" not in line:
+                    if info:
+                        break
+                    continue
+                k, v = (s.strip() for s in line.split(":", 1))
+                if k in ("model name", "cpu family", "model", "stepping", "flags") and k not in info:
+                    info[k] = v
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    flags = set(info.get("flags", "").split())
+    simd = [f for f in ("avx2", "fma", "avx512f", "avx512vl", "avx512dq") if f in flags]
+    return (f"{info.get('model name', 'unknown')} (family {info.get('cpu family', '?')}, "
+            f"model {info.get('model', '?')}, stepping {info.get('stepping', '?')}; "
+            f"{' '.join(simd)})")
 
 
 def plain_path(cx, w, m):
@@ -551,14 +563,17 @@ def main():
                                   "max": int(per_inst.max())},
         "failed_instances": int(np.count_nonzero(st)),
         "roofline": {
-            "bound": "mfma",
+            "bound": "fp64_valu",
             "compute": "fp64 (MI355X fp64 vector peak = fp64 matrix peak = 78.6 TFLOP/s; "
-                       "this kernel issues FP64 VALU fma)",
+                       "this kernel issues FP64 VALU fma, no MFMA)",
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
             "traffic": MEASURED_TRAFFIC_BYTES,
+            "traffic_source": "profiles/rk_ensemble_lorenz_2p20.md (rocprofv3 --pmc FETCH_SIZE "
+                              "and WRITE_SIZE runs of this workload; a constant here, not "
+                              "measured in this run)",
             "traffic_note": "rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (factors from a "
                             "1 GiB copy kernel); above the algorithmic bytes because the "
                             "longest-first schedule gathers each instance's inputs and scatters "
@@ -601,7 +616,7 @@ def main():
                                                  C.byref(du), C.byref(tnu), cx.sh)
             if rc != 0:
                 raise RuntimeError(rc)
-        wall_u, kern_u = time_launches(cx, run_u, 2, 1)
+        wall_u, kern_u = time_launches(cx, run_u, args.steps, args.warmup)
         cu = bu["counts"].cpu().numpy().astype(np.int64)
         tol8 = {"solves_per_s": n / wall_u, "ms_per_solve": wall_u * 1e3,
                 "accepted_steps_per_s": int(cu[:, 0].sum()) / wall_u,
@@ -774,9 +789,10 @@ def main():
                "frac_of_fp64_peak": int(cl.sum()) * FLOPS_ATTEMPT_LINEAR / kern_l / 1e12
                                     / PEAK_FP64_TFLOPS,
                "max_abs_err_vs_closed_form": float(np.abs(yl - ye).max()),
-               # rocprofv3 on linear_rk_kernel, see profiles/linear_mfma_65536.md
                "mfma_utilisation_rocprof": 0.78,
                "hbm_traffic_bytes_rocprof": 68.4e6 + 68.6e6,
+               "rocprof_source": "profiles/linear_mfma_65536.md (rocprofv3 --pmc runs of "
+                                 "this workload; constants here, not measured in this run)",
                "algorithmic_hbm_bytes": nl * LIN_N * 8 * 2 + nl * 24}
         if not args.no_cpu_baseline:
             O = baseline_oracle()
