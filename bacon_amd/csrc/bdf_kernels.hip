@@ -22,10 +22,6 @@ namespace bg {
 
 static constexpr uint32_t kBdfDefaultMaxSteps = 1000000u;
 
-// Per-wave scratch of the whole-wave (32-component) layout, in doubles: the
-// lower-order residual's finite-difference Jacobian [32 columns][32 rows],
-// then per lane that residual at y_n and the guess after the column loop.
-static constexpr uint32_t kBdfScratchDoubles = 32 * 32 + 2 * 64;
 
 // ---------------------------------------------------------------------------
 // cross-lane moves without LDS traffic (gfx950 lane layouts, measured):
@@ -237,6 +233,83 @@ struct RhsBrusselatorG {
     }
     // FLOPs of one full evaluation (all 2*CELLS components)
     static constexpr int FLOPS = CELLS * (6 + 4 + 4 + 5);
+
+    // Column colouring for the finite-difference Jacobian (Curtis, Powell &
+    // Reid 1974): row u_i depends on u_{i-1}, u_i, u_{i+1}, v_i and row v_i
+    // on v_{i-1}, v_i, v_{i+1}, u_i.  Colour u_j with j mod 4 and v_j with
+    // (j + 2) mod 4: the four members of every row's stencil then carry the
+    // four colours, so one evaluation per colour (and sign) yields one
+    // entry of every row (CELLS divisible by 4 keeps the periodic wrap
+    // consistent).
+    static constexpr int NC = 4;
+    // the column of colour k in row g's stencil
+    __device__ static int member(int g, int k) {
+        const bool isu = g < CELLS;
+        const int i = isu ? g : g - CELLS;
+        const int d = (k - i) & 3;
+        if (isu) {
+            const int c[4] = {g, (i + 1) % CELLS, CELLS + i, (i + CELLS - 1) % CELLS};
+            return c[d];
+        }
+        const int c[4] = {i, CELLS + (i + CELLS - 1) % CELLS, g, CELLS + (i + 1) % CELLS};
+        return c[d];
+    }
+    // row g's stencil in increasing column order
+    __device__ static void stencil(int g, int (&s)[4]) {
+        const bool isu = g < CELLS;
+        const int i = isu ? g : g - CELLS;
+        const int base = isu ? 0 : CELLS;
+        int a[4] = {base + (i + CELLS - 1) % CELLS, g, base + (i + 1) % CELLS,
+                    isu ? CELLS + i : i};
+        // sort four (a tiny network; compile-time unrolled)
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+            for (int y = 0; y < 3 - x; ++y)
+                if (a[y] > a[y + 1]) {
+                    const int tmp = a[y];
+                    a[y] = a[y + 1];
+                    a[y + 1] = tmp;
+                }
+#pragma unroll
+        for (int x = 0; x < 4; ++x) s[x] = a[x];
+    }
+    // f_g at the state the reference's column loop (bdf.rs:402-406) holds
+    // while it evaluates column c: components j < c already restored
+    // (`rest`, x + dt - 2 dt + dt), j > c original (`orig`), c itself
+    // perturbed (`pert`: x + dt on half 0, x + dt - 2 dt on half 1).  Every
+    // lane supplies its own three candidates; `own` receives the value of
+    // the lane's own component in that state.  A c outside the stencil
+    // gives the unperturbed state of that column.
+    template <class G>
+    __device__ static bool lane_eval_state(double t, double orig, double rest, double pert, int c,
+                                           const double *p, const G &grp, double &f,
+                                           double &own) {
+        static_assert(CELLS == 16, "state evaluation uses the DPP row layout of 16 cells");
+        const int g = grp.g;
+        const bool isu = g < CELLS;
+        const int i = isu ? g : g - CELLS;
+        const int base = isu ? 0 : CELLS;
+        const int jl = base + (i + CELLS - 1) % CELLS, jr = base + (i + 1) % CELLS;
+        const int jo = isu ? g + CELLS : g - CELLS;
+        auto pick = [&](int j, double o, double r, double q) {
+            return j == c ? q : (j < c ? r : o);
+        };
+        own = pick(g, orig, rest, pert);
+        const double left = pick(jl, dpp_f64<0x121>(orig), dpp_f64<0x121>(rest), dpp_f64<0x121>(pert));
+        const double right =
+            pick(jr, dpp_f64<0x12F>(orig), dpp_f64<0x12F>(rest), dpp_f64<0x12F>(pert));
+        const double other = pick(jo, xor16_f64(orig), xor16_f64(rest), xor16_f64(pert));
+        const double u = isu ? own : other, v = isu ? other : own;
+        const double A = p[0], B = p[1], alpha = p[2];
+        const double dx = 1.0 / CELLS;
+        const double cl = alpha / (dx * dx);
+        const double uuv = u * u * v;
+        const double lap = fma(-2.0, own, left) + right;
+        const double react = isu ? fma(-(B + 1.0), u, A + uuv) : fma(B, u, -uuv);
+        f = fma(cl, lap, react);
+        return true;
+    }
 };
 
 // quirk register as compile-time data (bdf.rs:403-407, 568, 622)
@@ -297,7 +370,6 @@ struct BdfLane {
     Grp<W> grp;
     int h = 0;    // half of the wave (SPLIT = 2)
     double *lds;  // this group's W x (W+1) transpose buffer, then 2 scalars
-    double *scr = nullptr;  // SPLIT = 2: this wave's kBdfScratchDoubles
     float *prow_f = nullptr;  // SPLIT = 2, FP32 inverse: the pivot row
 
     // The elimination's pivot row (W entries) in the inverse's arithmetic
@@ -528,66 +600,97 @@ struct BdfLane {
         return ok;
     }
 
+    // bdf.rs:390-411 for the whole-wave layout (SPLIT = 2): the
+    // finite-difference Jacobian by column colouring.  The reference's loop
+    // perturbs one column at a time (x_c + dt, then x_c + dt - 2 dt, then
+    // restores x_c as x_c + dt - 2 dt + dt, which may differ from x_c by a
+    // rounding), so column c is evaluated at a state whose components
+    // before c are restored and after c original.  Entry (g, c) depends only
+    // on row g's stencil, so one evaluation per colour (half 0 at + dt,
+    // half 1 at - dt) gives every row its entry of that colour, each lane at
+    // the state its own column c would see: the same inputs, hence the same
+    // bits, as the column loop.  Entries outside the stencil see identical
+    // inputs above and below: the central difference (intended) makes them
+    // exactly 0; the sum (as built, Q3) makes them (G + G) / (2 dt) at the
+    // unperturbed state of their column, which depends only on how many
+    // stencil members precede the column (at most five states).  Lane
+    // (h, g) writes the entries of its half row h into row g of the LDS
+    // buffer.  Returns the guess after the loop (every component restored).
+    template <bool LOWER>
+    __device__ __forceinline__ bool fd_coloured(double tt, double denom, double &guess) {
+        const double orig = guess;
+        const double xp = orig + dt, xm = xp - 2.0 * dt;
+        const double rest = xm + dt;  // the reference's restore (bdf.rs:405)
+        const double pert = h == 0 ? xp : xm;
+        const int g = grp.g;
+        double *row = lds + g * (W + 1);
+        if constexpr (Sem::fd_sign > 0.0) {
+            // as built: the dense part, from the unperturbed states
+            int s[4];
+            RHS::stencil(g, s);
+            double Gm[5];
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                // a column with exactly m stencil members before it, if the
+                // row has one (otherwise the value is not used)
+                const int rep = m == 0 ? 0 : s[m - 1] + 1;
+                double fx, own;
+                if (!RHS::lane_eval_state(tt, orig, rest, pert, rep, p, grp, fx, own)) return false;
+                Gm[m] = combine<LOWER>(fx, own);
+            }
+#pragma unroll
+            for (int j = 0; j < WJ; ++j) {
+                const int c = h * WJ + j;
+                const int m = (s[0] < c) + (s[1] < c) + (s[2] < c) + (s[3] < c);
+                double v = Gm[0];
+#pragma unroll
+                for (int q = 1; q < 5; ++q) v = m == q ? Gm[q] : v;
+                row[c] = (v + Sem::fd_sign * v) * denom;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < WJ; ++j) row[h * WJ + j] = 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < RHS::NC; ++k) {
+            const int c = RHS::member(g, k);
+            double fx, own;
+            if (!RHS::lane_eval_state(tt, orig, rest, pert, c, p, grp, fx, own)) return false;
+            const double r = combine<LOWER>(fx, own);
+            const double other = xor32_f64(r);
+            const double above = h == 0 ? r : other, below = h == 0 ? other : r;
+            if (c / WJ == h) row[c] = (above + Sem::fd_sign * below) * denom;
+        }
+        guess = rest;
+        return true;
+    }
+
     // bdf.rs:414-475; on success `guess` holds the corrected value.
-    // SPLIT = 2: both secants of a step start from guess = y_n at the same
-    // time, and the column loop's restores are the same in both, so every
-    // f(t_n, x +/- dt e_c) the lower-order loop would evaluate equals one of
-    // the higher-order loop's.  The higher-order call therefore also forms
-    // the lower-order residual's columns, its residual at y_n and the
-    // restored guess from those same values, in the same operations, and
-    // leaves them in the wave's scratch; the lower-order call reads them
-    // back instead of evaluating the right-hand side again.
     template <bool LOWER>
     __device__ __forceinline__ int secant(double &guess) {
         const double tt = t + Sem::theta * dt;
         const double denom = 1.0 / (2.0 * dt);
-        const int lane = grp.g + h * W;
         double F;
-        if constexpr (SPLIT == 2 && LOWER) {
-            // the columns and values left by secant<false> of this step
-            __threadfence_block();
-            F = scr[W * W + lane];
-            guess = scr[W * W + 64 + lane];
-#pragma unroll
-            for (int j = 0; j < WJ; ++j)
-                lds[grp.g * (W + 1) + h * WJ + j] = scr[(h * WJ + j) * W + grp.g];
-        } else {
         guess = y;
         double fx0;
         if (!f(tt, guess, fx0)) return BG_USER_ERROR;
         F = combine<LOWER>(fx0, guess);
-        if constexpr (SPLIT == 2) scr[W * W + lane] = combine<true>(fx0, guess);
+        if constexpr (SPLIT == 2) {
+            if (!fd_coloured<LOWER>(tt, denom, guess)) return BG_USER_ERROR;
+        } else {
         // bdf.rs:390-411: columns (G(x + dt e_c) +/- G(x - dt e_c)) / (2 dt);
         // lane g stores J(g, c) into row g of the group's LDS buffer
 #pragma unroll 1
         for (int c = 0; c < W; ++c) {
             const bool mine = (grp.g == c);
-            if constexpr (SPLIT == 2) {
-                // half 0 evaluates at x + dt e_c, half 1 at (x + dt e_c) - 2 dt e_c
-                const double xp = guess + dt, xm = xp - 2.0 * dt;
-                const double xg = mine ? (h == 0 ? xp : xm) : guess;
-                double fx;
-                if (!f(tt, xg, fx)) return BG_USER_ERROR;
-                const double r = combine<false>(fx, xg);
-                const double other = xor32_f64(r);
-                const double above = h == 0 ? r : other, below = h == 0 ? other : r;
-                const double rl = combine<true>(fx, xg);
-                const double other_l = xor32_f64(rl);
-                const double above_l = h == 0 ? rl : other_l, below_l = h == 0 ? other_l : rl;
-                if (mine) guess = xm + dt;  // the reference's restore (bdf.rs:405)
-                lds[grp.g * (W + 1) + c] = (above + Sem::fd_sign * below) * denom;
-                if (h == 0) scr[c * W + grp.g] = (above_l + Sem::fd_sign * below_l) * denom;
-            } else {
-                if (mine) guess += dt;
-                double above, below;
-                if (!residual<LOWER>(tt, guess, above)) return BG_USER_ERROR;
-                if (mine) guess -= 2.0 * dt;
-                if (!residual<LOWER>(tt, guess, below)) return BG_USER_ERROR;
-                if (mine) guess += dt;
-                lds[grp.g * (W + 1) + c] = (above + Sem::fd_sign * below) * denom;
-            }
+            if (mine) guess += dt;
+            double above, below;
+            if (!residual<LOWER>(tt, guess, above)) return BG_USER_ERROR;
+            if (mine) guess -= 2.0 * dt;
+            if (!residual<LOWER>(tt, guess, below)) return BG_USER_ERROR;
+            if (mine) guess += dt;
+            lds[grp.g * (W + 1) + c] = (above + Sem::fd_sign * below) * denom;
         }
-        if constexpr (SPLIT == 2) scr[W * W + 64 + lane] = guess;
         }
         __builtin_amdgcn_wave_barrier();
         ++n_lu;
@@ -743,8 +846,7 @@ struct BdfLane {
 // measured 20 % faster than the 2 that fit without spilling (4: slower).
 template <class RHS, int O, class Sem, class LuT, int SPLIT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
-    bdf_ensemble_kernel(const volatile bg_ensemble_desc *dv, uint32_t *__restrict__ counter,
-                        double *__restrict__ scratch) {
+    bdf_ensemble_kernel(const volatile bg_ensemble_desc *dv, uint32_t *__restrict__ counter) {
     constexpr int W = RHS::D;
     constexpr int GW = W * SPLIT;  // lanes per instance
     constexpr int GPW = 64 / GW;   // groups per wave
@@ -756,7 +858,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
     L.h = (lane % GW) / W;
     L.lds = lds_all[grp_id];
     // one wave per block; SPLIT = 2 holds one instance per wave
-    if constexpr (SPLIT == 2) L.scr = scratch + (uint64_t)blockIdx.x * kBdfScratchDoubles;
     if constexpr (SPLIT == 2 && sizeof(LuT) == sizeof(float)) {
         __shared__ float prow_all[W];
         L.prow_f = prow_all;
@@ -973,8 +1074,7 @@ bool bdf_supported(int method, int rhs, uint32_t dim) {
 }
 
 // Persistent grid (one wave per block, as many as are resident) and the
-// launch workspace: counter and descriptor copy, then for the whole-wave
-// layout kBdfScratchDoubles per block.
+// launch workspace: the work counter and the descriptor copy.
 template <class R, int O, class S, class F>
 static int bdf_launch_shape(const bg_ensemble_desc &d, uint64_t &grid, size_t &bytes) {
     constexpr int W = R::D;
@@ -989,7 +1089,7 @@ static int bdf_launch_shape(const bg_ensemble_desc &d, uint64_t &grid, size_t &b
     grid = (uint64_t)per_cu * (uint64_t)device_cu_count(dev);
     const uint64_t need = (d.n + GPW - 1) / GPW;
     if (grid > need) grid = need;
-    bytes = kLaunchWorkspaceBytes + (SP == 2 ? grid * kBdfScratchDoubles * sizeof(double) : 0);
+    bytes = kLaunchWorkspaceBytes;
     return BG_OK;
 }
 
@@ -1029,10 +1129,9 @@ int bdf_launch_ensemble(int method, int rhs, int semantics, int lu_fp32, const b
                          if (!ws_in) BG_HIP_CHECK(hipMallocAsync((void **)&ws, bytes, stream));
                          uint32_t *counter = (uint32_t *)ws;
                          bg_ensemble_desc *ddesc = (bg_ensemble_desc *)(ws + 256);
-                         double *scratch = SP == 2 ? (double *)(ws + kLaunchWorkspaceBytes) : nullptr;
                          bdf_setup_kernel<<<1, 64, 0, stream>>>(counter, ddesc, d);
                          bdf_ensemble_kernel<R, O, S, F, SP>
-                             <<<(uint32_t)grid, 64, 0, stream>>>(ddesc, counter, scratch);
+                             <<<(uint32_t)grid, 64, 0, stream>>>(ddesc, counter);
                          const hipError_t launched = hipGetLastError();
                          if (!ws_in) BG_HIP_CHECK(hipFreeAsync(ws, stream));
                          BG_HIP_CHECK(launched);
