@@ -15,8 +15,9 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbacon_gpu.so")
 ARCH = os.environ.get("BACON_GPU_ARCH", "gfx950")
 
-SOURCES = ["api.hip", "rk_kernels.hip", "bdf_kernels.hip", "adjoint_kernels.hip",
-           "linear_kernels.hip", "tune.hip", "graph.hip", "adams_kernels.hip"]
+SOURCES = ["bdf_kernels.hip", "bdf_systems.hip", "api.hip", "rk_kernels.hip",
+           "adjoint_kernels.hip", "linear_kernels.hip", "tune.hip", "graph.hip",
+           "adams_kernels.hip"]
 
 # -ffp-contract=off: every multiply-add in the kernels is an explicit fma(),
 # so results are reproducible operation by operation.

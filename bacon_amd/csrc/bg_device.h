@@ -206,6 +206,23 @@ struct RhsLotkaVolterra {
     }
 };
 
+// Robertson's chemical kinetics (stiff): params k1, k2, k3 (classically
+// 0.04, 3e7, 1e4); y0' = -k1 y0 + k3 y1 y2, y1' = k1 y0 - k3 y1 y2 - k2 y1^2,
+// y2' = k2 y1^2.  Separate products and sums (no fused operations).
+struct RhsRobertson {
+    static constexpr int D = 3, P = 3;
+    static constexpr int FLOPS = 10;
+    __device__ static bool eval(double, const double (&y)[3], const double *p, double (&f)[3]) {
+        const double a = p[0] * y[0];
+        const double b = p[2] * (y[1] * y[2]);
+        const double c = p[1] * (y[1] * y[1]);
+        f[0] = b - a;
+        f[1] = (a - b) - c;
+        f[2] = c;
+        return true;
+    }
+};
+
 struct RhsOscillator {
     static constexpr int D = 2, P = 0;
     __device__ static bool eval(double, const double (&y)[2], const double *, double (&f)[2]) {

@@ -97,6 +97,15 @@ int bdf_workspace_bytes(int method, int rhs, int semantics, int lu_fp32, const b
                         size_t *bytes);
 int bdf_launch_path(int method, int rhs, int semantics, uint32_t dim, BdfPathState *state,
                     double *path_t, double *path_y, uint32_t cap, hipStream_t stream);
+// the small systems (bdf_systems.hip), reached through the functions above
+bool bdf_sys_supported(int method, int rhs, uint32_t dim);
+int bdf_sys_workspace_bytes(int method, int rhs, int semantics, int lu_fp32,
+                            const bg_ensemble_desc &d, size_t *bytes);
+int bdf_sys_launch_ensemble(int method, int rhs, int semantics, int lu_fp32,
+                            const bg_ensemble_desc &d, hipStream_t stream, void *ws_in,
+                            size_t ws_bytes);
+int bdf_sys_launch_path(int method, int rhs, int semantics, uint32_t dim, BdfPathState *state,
+                        double *path_t, double *path_y, uint32_t cap, hipStream_t stream);
 
 // adjoint (adjoint_kernels.hip)
 int rk_launch_adjoint(int method, int rhs, const bg_adjoint_desc &d, hipStream_t stream);

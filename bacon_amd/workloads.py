@@ -117,6 +117,22 @@ def brusselator(n: int = 131072, first: int = 0, cells: int = 16):
                 dt_max=0.1)
 
 
+def robertson(n: int = 4096, first: int = 0):
+    """Robertson's stiff chemical kinetics: k1 = 0.04 x logU[0.5, 2],
+    k2 = 3e7 x logU[0.5, 2], k3 = 1e4 x logU[0.5, 2], y0 = (1, 0, 0),
+    t in [0, 10], tol 1e-6, dt in [1e-10, 5e-4] (the BDF bootstrap is
+    explicit RK4, and BDF doubles dt up to dt_max after every accurate step:
+    dt_max 5e-4 keeps that bootstrap stable across the rate constants)
+    (workload index 5)."""
+    s = Stream(5, n, first)
+    params = np.stack([0.04 * s.log_uniform(0.5, 2.0), 3e7 * s.log_uniform(0.5, 2.0),
+                       1e4 * s.log_uniform(0.5, 2.0)])
+    y0 = np.stack([np.ones(n), np.zeros(n), np.zeros(n)])
+    return dict(rhs="robertson", dim=3, y0=y0, params=params, t0=0.0,
+                t_end=np.full(n, 10.0), tol=np.full(n, 1e-6), dt_min=1e-10,
+                dt_max=5e-4)
+
+
 def linear_shared(n: int = 65536, first: int = 0, N: int = 128):
     """Linear y' = A y + b with one shared A = -(Q D Q^T): Q the sign-fixed
     orthogonal factor of a seeded N(0,1) matrix, D log-uniform in [0.1,10],

@@ -84,6 +84,7 @@ extern "C" {
 #define BG_RHS_LOTKA_VOLTERRA 7 /* params: alpha, beta, gamma, delta; dim 2  */
 #define BG_RHS_OSCILLATOR 8     /* y0' = y1, y1' = -y0 (ivp.rs cos_deriv)    */
 #define BG_RHS_FAIL_AFTER 9     /* y' = y; UserError once t >= params[0]     */
+#define BG_RHS_ROBERTSON 10     /* stiff kinetics; params: k1, k2, k3; dim 3 */
 
 /* ======================================================================== */
 /* Single-instance builder + iterator                                       */

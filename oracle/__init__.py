@@ -43,7 +43,7 @@ PROFILE_BACON, PROFILE_DEVICE = 0, 1
 RHS = {
     "exp": 0, "neg": 1, "quadratic": 2, "cos": 3, "lorenz": 4, "linear": 5,
     "brusselator": 6, "lotka_volterra": 7, "oscillator": 8, "fail_after": 9,
-    "lv_sens": 10,
+    "lv_sens": 10, "robertson": 11,
 }
 
 _dp = C.POINTER(C.c_double)

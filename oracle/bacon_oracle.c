@@ -182,6 +182,17 @@ int32_t or_rhs_eval(int32_t rhs, int32_t prof, int32_t dim, double t,
         f[0] = y[1];
         f[1] = -y[0];
         return OR_OK;
+    case OR_RHS_ROBERTSON: {
+        /* Robertson's kinetics; separate products and sums in both profiles */
+        if (dim != 3) return OR_INVALID_ARGUMENT;
+        const double a = pr[0] * y[0];
+        const double b = pr[2] * (y[1] * y[2]);
+        const double c = pr[1] * (y[1] * y[1]);
+        f[0] = b - a;
+        f[1] = (a - b) - c;
+        f[2] = c;
+        return OR_OK;
+    }
     case OR_RHS_FAIL_AFTER:
         if (t >= pr[0]) return OR_USER_ERROR;
         for (int i = 0; i < dim; ++i) f[i] = y[i];

@@ -85,7 +85,8 @@ enum {
     OR_RHS_OSCILLATOR = 8,   /* y0' = y1, y1' = -y0 (ivp.rs cos_deriv)        */
     OR_RHS_FAIL_AFTER = 9,   /* y' = y, UserError once t >= params[0]         */
     OR_RHS_LV_SENS = 10,     /* 14-dim forward-sensitivity system of LV       */
-    OR_RHS_COUNT = 11
+    OR_RHS_ROBERTSON = 11,   /* stiff kinetics, params k1, k2, k3; dim 3      */
+    OR_RHS_COUNT = 12
 };
 
 #define OR_MAX_DIM 128

@@ -10,8 +10,10 @@ oracle), used to pin the oracle's "intended" solvers:
                       DOP853 rtol=atol=1e-13, states at t_k = 0.75 k, k=1..20
   bruss_golden.npz  : 8 Brusselator (16 cells) instances, Radau rtol=atol=1e-12,
                       states at t = 1, 10
+  robertson_golden.npz : 16 Robertson instances of the stiff-kinetics workload,
+                      Radau rtol=1e-12, atol=1e-14, states at t = 10
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [robertson]
 """
 from __future__ import annotations
 
@@ -45,7 +47,23 @@ def bruss_f(t, y, A, B, alpha):
     return np.concatenate([A + uuv - (B + 1) * u + c * lu, B * u - uuv + c * lv])
 
 
+def robertson_f(t, y, k1, k2, k3):
+    return [-k1 * y[0] + k3 * y[1] * y[2], k1 * y[0] - k3 * y[1] * y[2] - k2 * y[1] ** 2,
+            k2 * y[1] ** 2]
+
+
+def robertson():
+    w = W.robertson(16)
+    out = np.zeros((3, 16))
+    for i in range(16):
+        s = solve_ivp(robertson_f, (0.0, 10.0), w["y0"][:, i], method="Radau", rtol=1e-12,
+                      atol=1e-14, args=tuple(w["params"][:, i]))
+        out[:, i] = s.y[:, -1]
+    np.savez(os.path.join(HERE, "robertson_golden.npz"), params=w["params"], y=out)
+
+
 def main():
+    robertson()
     w = W.lorenz_plain(32)
     times = np.array([0.5, 1.0, 2.0, 5.0])
     out = np.zeros((32, times.size, 3))
@@ -79,4 +97,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["robertson"]:
+        robertson()
+    else:
+        main()

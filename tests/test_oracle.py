@@ -439,6 +439,20 @@ def test_brusselator_bdf6_vs_radau(orc):
         assert np.abs(r["y"] - g["y"][i, 0]).max() < 1e-4
 
 
+@pytest.mark.parametrize("method", [O.BDF6, O.BDF2])
+def test_robertson_bdf_vs_radau(orc, method):
+    """BDF at dimension 3 on Robertson's stiff kinetics against scipy Radau
+    (rtol 1e-12) at t = 10: within 1e-5 (tol 1e-6); pins the Robertson
+    right-hand side and the dimension-generic BDF loop."""
+    g = np.load("tests/golden/robertson_golden.npz")
+    w = W.robertson(16)
+    assert np.array_equal(w["params"], g["params"])
+    o = O.solve_ensemble(method, "robertson", w["y0"], w["params"], w["t_end"], w["tol"],
+                         dt_min=w["dt_min"], dt_max=w["dt_max"], semantics=O.INTENDED)
+    assert np.all(o["status"] == O.OK)
+    assert np.abs(o["y"] - g["y"]).max() <= 1e-5
+
+
 # ---------------------------------------------------------------- profiles
 def test_device_profile_agrees_with_bacon_profile(orc):
     """The kernels' arithmetic (fused multiply-adds, the squared-norm
