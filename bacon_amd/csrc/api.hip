@@ -841,6 +841,11 @@ extern "C" int bg_bdf_solve_device(int method, int rhs, int semantics, int lu_bi
     return bdf_launch_ensemble(method, rhs, semantics, lu_bits == 32, *d, (hipStream_t)stream);
 }
 
+extern "C" int bg_bdf_inverse_device(uint32_t dim, uint64_t count, const double *a, double *inv,
+                                     int32_t *branch, int first, void *stream) {
+    return bdf_launch_inverse(dim, count, a, inv, branch, first, (hipStream_t)stream);
+}
+
 extern "C" int bg_adams_solve_device(int method, int rhs, int semantics,
                                      const bg_ensemble_desc *d, void *stream) {
     if (!d || !is_adams(method)) return BG_ERR_INVALID_ARGUMENT;

@@ -698,6 +698,179 @@ struct BdfLane {
         return true;
     }
 
+    // bdf.rs:390-411: the finite-difference Jacobian into the LDS rows,
+    // starting from guess = y; leaves the guess as the reference's column
+    // loop does.  False on a UserError.
+    template <bool LOWER>
+    __device__ __forceinline__ bool fd_jacobian(double tt, double denom, double &guess) {
+        guess = y;
+        if constexpr (SPLIT == 2) {
+            if (!fd_coloured<LOWER>(tt, denom, guess)) return false;
+        } else {
+        // columns (G(x + dt e_c) +/- G(x - dt e_c)) / (2 dt); lane g stores
+        // J(g, c) into row g of the group's LDS buffer
+#pragma unroll 1
+        for (int c = 0; c < W; ++c) {
+            if (c >= D) {  // padding column: e_c
+                lds[grp.g * (W + 1) + c] = grp.g == c ? 1.0 : 0.0;
+                continue;
+            }
+            const bool mine = (grp.g == c);
+            if (mine) guess += dt;
+            double above, below;
+            if (!residual<LOWER>(tt, guess, above)) return false;
+            if (mine) guess -= 2.0 * dt;
+            if (!residual<LOWER>(tt, guess, below)) return false;
+            if (mine) guess += dt;
+            lds[grp.g * (W + 1) + c] = (above + Sem::fd_sign * below) * denom;
+        }
+        }
+        __builtin_amdgcn_wave_barrier();
+        return true;
+    }
+
+    // ---- the rest of the inverse cascade (bdf.rs:429-444) -----------------
+    // nalgebra tries partial-pivot LU, then full-pivot LU, then QR, and
+    // reports SingularMatrix only when all three meet an exact zero pivot.
+    // The fast path is the partial-pivot elimination above; when it finds
+    // no pivot, these two run on the same matrix (rebuilt in the LDS rows by
+    // fd_jacobian).  They are rare, so they work in LDS with rolled loops
+    // (few registers) and keep their per-lane temporaries in private memory.
+    // Both halves of a whole-wave group compute the same values; every lane
+    // ends holding its part of row g of the inverse in jinv.
+    __device__ __forceinline__ double &mat(int r, int c) const { return lds[r * (W + 1) + c]; }
+
+    // row g of the inverse, given as the matrix M whose row qinv(g) holds
+    // row g with column j at position col(j); col(r) is in mat(r, W).
+    __device__ __forceinline__ void gather_permuted_row() {
+        const int g = grp.g;
+        int src = 0;
+#pragma unroll 1
+        for (int r = 0; r < W; ++r) src = ((int)mat(r, W) == g) ? r : src;
+        double row[WJ];
+#pragma unroll
+        for (int q = 0; q < WJ; ++q) {
+            int cq = (int)mat(h * WJ + q, W);
+            cq = cq < 0 ? 0 : (cq >= W ? W - 1 : cq);
+            row[q] = mat(src, cq);
+        }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int q = 0; q < WJ; ++q) jinv[q] = row[q];
+    }
+
+    // Full-pivot Gauss-Jordan (exchange steps): step k pivots on the largest
+    // |a| over the unused rows and columns (first column, then lowest row,
+    // on ties) and fails on an exact zero, like FullPivLU's try_inverse.
+    __device__ __forceinline__ bool invert_full_pivot() {
+        const int g = grp.g;
+        bool used = false;
+        uint32_t ucols = 0;  // columns already pivoted (the same in every lane)
+        int my_col = -1;
+#pragma unroll 1
+        for (int k = 0; k < W; ++k) {
+            double best = -1.0;
+            int bc = 0;
+            if (!used) {
+#pragma unroll 1
+                for (int c = 0; c < W; ++c) {
+                    const double v = fabs(mat(g, c));
+                    if (!((ucols >> c) & 1u) && v > best) {
+                        best = v;
+                        bc = c;
+                    }
+                }
+            }
+            const double gmax = grp.max(best);
+            if (!(gmax > 0.0)) return false;
+            const uint32_t key = grp.max_u32((!used && best == gmax) ? (uint32_t)(63 - g) : 0u);
+            const int p = 63 - (int)key;
+            const int q = (int)grp.bcast((double)bc, p);
+            const double ip = 1.0 / mat(p, q);
+            __builtin_amdgcn_wave_barrier();
+            if (g == p) {
+#pragma unroll 1
+                for (int j = 0; j < W; ++j) mat(p, j) = j == q ? ip : mat(p, j) * ip;
+                used = true;
+                my_col = q;
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (g != p) {
+                const double fq = mat(g, q);
+#pragma unroll 1
+                for (int j = 0; j < W; ++j)
+                    if (j != q) mat(g, j) = fma(-fq, mat(p, j), mat(g, j));
+                mat(g, q) = -fq * ip;
+            }
+            ucols |= 1u << q;
+            __builtin_amdgcn_wave_barrier();
+        }
+        mat(g, W) = (double)my_col;
+        __builtin_amdgcn_wave_barrier();
+        gather_permuted_row();
+        return true;
+    }
+
+    // Householder QR (reflections chosen away from cancellation), failing on
+    // an exact zero on R's diagonal like QR's try_inverse.  Reflection k is
+    // kept below the diagonal of column k, scaled to a unit leading entry,
+    // with its factor in mat(k, W).  Row g of the inverse R^-1 Q^T is
+    // (Q w)^T with R^T w = e_g, solved by each lane on its own.
+    __device__ __forceinline__ bool invert_qr() {
+        const int g = grp.g;
+#pragma unroll 1
+        for (int k = 0; k < W; ++k) {
+            const double x = g >= k ? mat(g, k) : 0.0;
+            const double nrm = sqrt(grp.sum(x * x));
+            const double akk = mat(k, k);
+            const double alpha = akk >= 0.0 ? -nrm : nrm;
+            const double vk = akk - alpha;
+            double v = g > k ? x : (g == k ? vk : 0.0);
+            const double vv = grp.sum(v * v);
+            const bool skip = nrm == 0.0 || vv == 0.0;
+            const double beta = skip ? 0.0 : 2.0 / vv;
+#pragma unroll 1
+            for (int c = k; c < W; ++c) {
+                const double dot = grp.sum(v * mat(g, c));
+                __builtin_amdgcn_wave_barrier();
+                if (g >= k) mat(g, c) = fma(-(beta * dot), v, mat(g, c));
+                __builtin_amdgcn_wave_barrier();
+            }
+            // unit leading entry: v / vk, factor beta vk^2
+            const double sv = skip ? 0.0 : v / vk;
+            if (g > k) mat(g, k) = sv;
+            if (g == k) mat(k, W) = skip ? 0.0 : beta * vk * vk;
+            __builtin_amdgcn_wave_barrier();
+        }
+        bool ok = true;
+#pragma unroll 1
+        for (int k = 0; k < W; ++k) ok = ok && mat(k, k) != 0.0;
+        if (!ok) return false;
+        double w[W];
+#pragma unroll 1
+        for (int i = 0; i < W; ++i) {  // R^T w = e_g, forward
+            double s = i == g ? 1.0 : 0.0;
+#pragma unroll 1
+            for (int l = 0; l < i; ++l) s = fma(-mat(l, i), w[l], s);
+            w[i] = s / mat(i, i);
+        }
+#pragma unroll 1
+        for (int k = W - 1; k >= 0; --k) {  // w <- H_k w, last reflection first
+            const double bk = mat(k, W);
+            double dot = w[k];
+#pragma unroll 1
+            for (int i = k + 1; i < W; ++i) dot = fma(mat(i, k), w[i], dot);
+            const double sc = bk * dot;
+            w[k] -= sc;
+#pragma unroll 1
+            for (int i = k + 1; i < W; ++i) w[i] = fma(-sc, mat(i, k), w[i]);
+        }
+#pragma unroll
+        for (int q = 0; q < WJ; ++q) jinv[q] = w[h * WJ + q];
+        __builtin_amdgcn_wave_barrier();
+        return true;
+    }
+
     // bdf.rs:414-475; on success `guess` holds the corrected value.
     template <bool LOWER>
     __device__ __forceinline__ int secant(double &guess) {
@@ -708,30 +881,17 @@ struct BdfLane {
         double fx0;
         if (!f(tt, guess, fx0)) return BG_USER_ERROR;
         F = combine<LOWER>(fx0, guess);
-        if constexpr (SPLIT == 2) {
-            if (!fd_coloured<LOWER>(tt, denom, guess)) return BG_USER_ERROR;
-        } else {
-        // bdf.rs:390-411: columns (G(x + dt e_c) +/- G(x - dt e_c)) / (2 dt);
-        // lane g stores J(g, c) into row g of the group's LDS buffer
-#pragma unroll 1
-        for (int c = 0; c < W; ++c) {
-            if (c >= D) {  // padding column: e_c
-                lds[grp.g * (W + 1) + c] = grp.g == c ? 1.0 : 0.0;
-                continue;
-            }
-            const bool mine = (grp.g == c);
-            if (mine) guess += dt;
-            double above, below;
-            if (!residual<LOWER>(tt, guess, above)) return BG_USER_ERROR;
-            if (mine) guess -= 2.0 * dt;
-            if (!residual<LOWER>(tt, guess, below)) return BG_USER_ERROR;
-            if (mine) guess += dt;
-            lds[grp.g * (W + 1) + c] = (above + Sem::fd_sign * below) * denom;
-        }
-        }
-        __builtin_amdgcn_wave_barrier();
+        if (!fd_jacobian<LOWER>(tt, denom, guess)) return BG_USER_ERROR;
         ++n_lu;
-        if (!invert()) return BG_SINGULAR_MATRIX;
+        if (!invert()) {
+            // bdf.rs:432-441: full-pivot LU, then QR, each on the Jacobian
+            // rebuilt from the same evaluations
+            if (!fd_jacobian<LOWER>(tt, denom, guess)) return BG_USER_ERROR;
+            if (!invert_full_pivot()) {
+                if (!fd_jacobian<LOWER>(tt, denom, guess)) return BG_USER_ERROR;
+                if (!invert_qr()) return BG_SINGULAR_MATRIX;
+            }
+        }
         double s = neg_jinv_times(F);
         guess += s;
 #pragma unroll 1
@@ -1061,6 +1221,61 @@ __global__ void bdf_setup_kernel(uint32_t *counter, bg_ensemble_desc *dst, const
         *counter = 0u;
         *dst = d;
     }
+}
+
+// ---------------------------------------------------------------------------
+// The corrector's inverse cascade on caller matrices (bg_bdf_inverse_device):
+// one matrix per group, W x W row-major in `a`; lane (h, g) stages row g in
+// the group's LDS rows, runs partial-pivot Gauss-Jordan, and on failure the
+// full-pivot and QR members on the reloaded matrix, exactly as the secant
+// does (from member `first` on: 0 the whole cascade, 1 full pivot, 2 QR);
+// it writes its part of row g of the inverse and the member that answered
+// (0, 1, 2; -1: SingularMatrix).
+// ---------------------------------------------------------------------------
+template <int W_>
+struct InverseOnly {  // a right-hand side in name only: the lane layout of W_
+    static constexpr int D = W_, W = W_, P = 0;
+};
+
+template <int W, int SPLIT>
+__global__ void __launch_bounds__(64)
+    bdf_inverse_kernel(const double *__restrict__ a, double *__restrict__ inv,
+                       int32_t *__restrict__ branch, uint64_t count, int first) {
+    constexpr int GW = W * SPLIT, GPW = 64 / GW;
+    __shared__ double lds_all[GPW][W * (W + 1) + 2];
+    const int lane = __lane_id();
+    const int grp_id = lane / GW;
+    BdfLane<InverseOnly<W>, 7, BdfSem<false>, double, SPLIT> L;
+    L.grp.g = lane % W;
+    L.h = (lane % GW) / W;
+    L.lds = lds_all[grp_id];
+    const uint64_t m = (uint64_t)blockIdx.x * GPW + grp_id;
+    const bool live = m < count;
+    const uint64_t mm = live ? m : 0;  // a spare group inverts matrix 0 and writes nothing
+    auto stage = [&]() {
+        for (int c = 0; c < W; ++c)
+            L.lds[L.grp.g * (W + 1) + c] = a[(mm * W + L.grp.g) * W + c];
+        __builtin_amdgcn_wave_barrier();
+    };
+    stage();
+    int br = first;
+    bool ok = false;
+    if (first == 0) ok = L.invert();
+    if (!ok && first <= 1) {
+        stage();
+        br = 1;
+        ok = L.invert_full_pivot();
+    }
+    if (!ok) {
+        stage();
+        br = 2;
+        ok = L.invert_qr();
+    }
+    if (!ok) br = -1;
+    if (!live) return;
+#pragma unroll
+    for (int q = 0; q < L.WJ; ++q) inv[(mm * W + L.grp.g) * W + L.h * L.WJ + q] = L.jinv[q];
+    if (L.grp.g == 0 && L.h == 0) branch[mm] = br;
 }
 
 // ---------------------------------------------------------------------------

@@ -77,6 +77,28 @@ int bdf_launch_ensemble(int method, int rhs, int semantics, int lu_fp32, const b
     return bdf_sys_launch_ensemble(method, rhs, semantics, lu_fp32, d, stream, ws_in, ws_bytes);
 }
 
+int bdf_launch_inverse(uint32_t dim, uint64_t count, const double *a, double *inv,
+                       int32_t *branch, int first, hipStream_t stream) {
+    if (first < 0 || first > 2) return BG_ERR_INVALID_ARGUMENT;
+    if (count == 0) return BG_OK;
+    if (!a || !inv || !branch || count > 0x7fffffffull) return BG_ERR_INVALID_ARGUMENT;
+    auto run = [&](auto kern, int per_block) -> int {
+        const uint64_t blocks = (count + per_block - 1) / per_block;
+        kern<<<(uint32_t)blocks, 64, 0, stream>>>(a, inv, branch, count, first);
+        BG_HIP_CHECK(hipGetLastError());
+        return BG_OK;
+    };
+    switch (dim) {
+    case 1: return run(bdf_inverse_kernel<1, 1>, 64);
+    case 2: return run(bdf_inverse_kernel<2, 1>, 32);
+    case 4: return run(bdf_inverse_kernel<4, 1>, 16);
+    case 8: return run(bdf_inverse_kernel<8, 1>, 8);
+    case 16: return run(bdf_inverse_kernel<16, 1>, 4);
+    case 32: return run(bdf_inverse_kernel<32, 2>, 1);
+    default: return BG_ERR_UNSUPPORTED;
+    }
+}
+
 int bdf_launch_path(int method, int rhs, int semantics, uint32_t dim, BdfPathState *state,
                     double *path_t, double *path_y, uint32_t cap, hipStream_t stream) {
     const int rc = visit_bdf(method, rhs, semantics, dim, 0,

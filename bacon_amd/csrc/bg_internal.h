@@ -97,6 +97,9 @@ int bdf_workspace_bytes(int method, int rhs, int semantics, int lu_fp32, const b
                         size_t *bytes);
 int bdf_launch_path(int method, int rhs, int semantics, uint32_t dim, BdfPathState *state,
                     double *path_t, double *path_y, uint32_t cap, hipStream_t stream);
+// the corrector's Jacobian inverse cascade on caller matrices
+int bdf_launch_inverse(uint32_t dim, uint64_t count, const double *a, double *inv,
+                       int32_t *branch, int first, hipStream_t stream);
 // the small systems (bdf_systems.hip), reached through the functions above
 bool bdf_sys_supported(int method, int rhs, uint32_t dim);
 int bdf_sys_workspace_bytes(int method, int rhs, int semantics, int lu_fp32,

@@ -322,6 +322,20 @@ void bg_graph_destroy(bg_graph *g);
 int bg_bdf_solve_device(int method, int rhs, int semantics, int lu_bits,
                         const bg_ensemble_desc *d, void *stream);
 
+/* The BDF corrector's Jacobian inverse (bdf.rs:429-444) on `count` caller
+ * matrices a[count][dim][dim] (row-major, device memory), dim in {1, 2, 4,
+ * 8, 16, 32}: partial-pivot Gauss-Jordan, then full-pivot Gauss-Jordan,
+ * then Householder QR, each tried when the one before meets an exact zero
+ * pivot (nalgebra's lu / full_piv_lu / qr try_inverse order).  `first`
+ * starts the cascade at member 0, 1 or 2 (0: the whole cascade, as the
+ * corrector runs it).  inv gets the inverses (same layout) and
+ * branch[count] the member that answered: 0, 1, 2, or -1 for
+ * SingularMatrix.  No counterpart of its own in the reference: the code the
+ * BDF kernels run inside the Broyden corrector, exposed for batched use and
+ * checking.  Asynchronous on `stream`. */
+int bg_bdf_inverse_device(uint32_t dim, uint64_t count, const double *a, double *inv,
+                          int32_t *branch, int first, void *stream);
+
 /* Batched Adams-Bashforth-Moulton (Adams5 / Adams3) in one persistent
  * launch on `stream` (asynchronous).  Semantics: BG_AS_BUILT keeps the
  * reference's derivative history, which misses the first Adams step after
