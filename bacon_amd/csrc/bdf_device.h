@@ -92,6 +92,22 @@ __device__ __forceinline__ float readlane_lu(float v, int l) {
 __device__ __forceinline__ double xor32_lu(double v) { return xor32_f64(v); }
 __device__ __forceinline__ float xor32_lu(float v) { return xor32_f32(v); }
 
+// 1 / a for a pivot of the whole-wave elimination: the hardware reciprocal
+// refined by Newton steps (two for double, one for float), within an ulp or
+// two of the correctly rounded quotient, in 3-5 instructions instead of the
+// IEEE division's ten.  The inverse is an approximation of the Jacobian's
+// anyway (the Broyden iteration only needs a nearby matrix); a is never 0
+// here (the pivot key rejected zeros).
+__device__ __forceinline__ double pivot_recip(double a) {
+    double r = __builtin_amdgcn_rcp(a);
+    r = fma(r, fma(-a, r, 1.0), r);
+    return fma(r, fma(-a, r, 1.0), r);
+}
+__device__ __forceinline__ float pivot_recip(float a) {
+    const float r = __builtin_amdgcn_rcpf(a);
+    return fmaf(r, fmaf(-a, r, 1.0f), r);
+}
+
 // ---------------------------------------------------------------------------
 // group collectives (every lane of the group active)
 // ---------------------------------------------------------------------------
@@ -120,10 +136,10 @@ struct Grp {
     // all-reduce maximum of unsigned keys (W == 32: one DPP max per stage)
     __device__ __forceinline__ uint32_t max_u32(uint32_t v) const {
         if constexpr (W == 32) {
-            v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false));
-            v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false));
-            v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, false));
-            v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xF, 0xF, false));
+            v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, true));
+            v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, true));
+            v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, true));
+            v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, true));
             v = __builtin_elementwise_max(v, xor16_u32(v));
             return v;
         } else {
@@ -550,7 +566,7 @@ struct BdfLane {
             if (piv) {
                 used = true;
                 my_col = k;
-                my_inv = (LuT)1 / agk;
+                my_inv = pivot_recip(agk);
 #pragma unroll
                 for (int j = 0; j < WJ; ++j) prow[h * WJ + j] = b[j];
                 // column k of the inverse: pivot * (1 / pivot)
