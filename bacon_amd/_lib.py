@@ -163,6 +163,10 @@ SIGNATURES = [
      [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.c_void_p]),
     ("bg_bdf_solve_device", C.c_int,
      [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.c_void_p]),
+    ("bg_bdf_autotune_device", C.c_int,
+     [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.POINTER(Tuning), C.c_void_p]),
+    ("bg_bdf_solve_tuned_device", C.c_int,
+     [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.POINTER(Tuning), C.c_void_p]),
     ("bg_bdf_inverse_device", C.c_int,
      [C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
     ("bg_adams_solve_device", C.c_int,

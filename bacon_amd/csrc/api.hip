@@ -841,6 +841,22 @@ extern "C" int bg_bdf_solve_device(int method, int rhs, int semantics, int lu_bi
     return bdf_launch_ensemble(method, rhs, semantics, lu_bits == 32, *d, (hipStream_t)stream);
 }
 
+extern "C" int bg_bdf_autotune_device(int method, int rhs, int semantics, int lu_bits,
+                                      const bg_ensemble_desc *d, bg_tuning *out, void *stream) {
+    if (!d || !out || !is_bdf(method) || (lu_bits != 32 && lu_bits != 64))
+        return BG_ERR_INVALID_ARGUMENT;
+    return bdf_autotune(method, rhs, semantics, lu_bits == 32, *d, *out, (hipStream_t)stream);
+}
+
+extern "C" int bg_bdf_solve_tuned_device(int method, int rhs, int semantics, int lu_bits,
+                                         const bg_ensemble_desc *d, const bg_tuning *tuning,
+                                         void *stream) {
+    if (!d || !tuning || !is_bdf(method) || (lu_bits != 32 && lu_bits != 64))
+        return BG_ERR_INVALID_ARGUMENT;
+    return bdf_solve_tuned(method, rhs, semantics, lu_bits == 32, *d, *tuning,
+                           (hipStream_t)stream);
+}
+
 extern "C" int bg_bdf_inverse_device(uint32_t dim, uint64_t count, const double *a, double *inv,
                                      int32_t *branch, int first, void *stream) {
     return bdf_launch_inverse(dim, count, a, inv, branch, first, (hipStream_t)stream);

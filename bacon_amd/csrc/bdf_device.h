@@ -1302,10 +1302,13 @@ struct BTag {
     using type = T;
 };
 
-// Persistent grid (one wave per block, as many as are resident) and the
-// launch workspace: the work counter and the descriptor copy.
+// Persistent grid (one wave per block, as many as are resident, at most
+// cap_per_cu per CU when cap_per_cu > 0) and the launch workspace: the work
+// counter and the descriptor copy.  *occupancy, if given, receives the
+// resident blocks per CU before the cap.
 template <class R, int O, class S, class F>
-static int bdf_launch_shape(const bg_ensemble_desc &d, uint64_t &grid, size_t &bytes) {
+static int bdf_launch_shape(const bg_ensemble_desc &d, uint64_t &grid, size_t &bytes,
+                            uint32_t cap_per_cu = 0, int *occupancy = nullptr) {
     constexpr int W = R::W;
     constexpr int SP = W == 32 ? 2 : 1;  // a 32-component instance takes the whole wave
     constexpr int GPW = 64 / (W * SP);
@@ -1314,6 +1317,8 @@ static int bdf_launch_shape(const bg_ensemble_desc &d, uint64_t &grid, size_t &b
     int per_cu = 0;
     BG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &per_cu, bdf_ensemble_kernel<R, O, S, F, SP>, 64, 0));
+    if (occupancy) *occupancy = per_cu;
+    if (cap_per_cu && (int)cap_per_cu < per_cu) per_cu = (int)cap_per_cu;
     if (per_cu < 1) per_cu = 1;
     grid = (uint64_t)per_cu * (uint64_t)device_cu_count(dev);
     const uint64_t need = (d.n + GPW - 1) / GPW;
@@ -1324,14 +1329,14 @@ static int bdf_launch_shape(const bg_ensemble_desc &d, uint64_t &grid, size_t &b
 
 template <class R, int O, class S, class F>
 static int bdf_launch_one(const bg_ensemble_desc &d, hipStream_t stream, void *ws_in,
-                          size_t ws_bytes) {
+                          size_t ws_bytes, uint32_t cap_per_cu = 0) {
     if (d.n == 0) return BG_OK;
     if (d.n > 0x7fffffffull || !d.y0) return BG_ERR_INVALID_ARGUMENT;
     if (R::P > 0 && (!d.params || d.n_params < (uint32_t)R::P)) return BG_ERR_INVALID_ARGUMENT;
     constexpr int SP = R::W == 32 ? 2 : 1;
     uint64_t grid = 0;
     size_t bytes = 0;
-    const int rc = bdf_launch_shape<R, O, S, F>(d, grid, bytes);
+    const int rc = bdf_launch_shape<R, O, S, F>(d, grid, bytes, cap_per_cu);
     if (rc != BG_OK) return rc;
     if (ws_in && ws_bytes < bytes) return BG_ERR_INVALID_ARGUMENT;
     char *ws = (char *)ws_in;

@@ -49,7 +49,7 @@ bool bdf_supported(int method, int rhs, uint32_t dim) {
 }
 
 int bdf_workspace_bytes(int method, int rhs, int semantics, int lu_fp32, const bg_ensemble_desc &d,
-                        size_t *bytes) {
+                        size_t *bytes, int *occupancy) {
     const int rc = visit_bdf(method, rhs, semantics, d.dim, lu_fp32,
                              [&](auto rtag, auto otag, auto stag, auto ftag) -> int {
                                  using R = typename decltype(rtag)::type;
@@ -57,24 +57,27 @@ int bdf_workspace_bytes(int method, int rhs, int semantics, int lu_fp32, const b
                                  using F = typename decltype(ftag)::type;
                                  constexpr int O = decltype(otag)::value;
                                  uint64_t grid = 0;
-                                 return bdf_launch_shape<R, O, S, F>(d, grid, *bytes);
+                                 return bdf_launch_shape<R, O, S, F>(d, grid, *bytes, 0,
+                                                                     occupancy);
                              });
     if (rc != BG_ERR_UNSUPPORTED) return rc;
-    return bdf_sys_workspace_bytes(method, rhs, semantics, lu_fp32, d, bytes);
+    return bdf_sys_workspace_bytes(method, rhs, semantics, lu_fp32, d, bytes, occupancy);
 }
 
 int bdf_launch_ensemble(int method, int rhs, int semantics, int lu_fp32, const bg_ensemble_desc &d,
-                        hipStream_t stream, void *ws_in, size_t ws_bytes) {
+                        hipStream_t stream, void *ws_in, size_t ws_bytes, uint32_t cap_per_cu) {
     const int rc = visit_bdf(method, rhs, semantics, d.dim, lu_fp32,
                              [&](auto rtag, auto otag, auto stag, auto ftag) -> int {
                                  using R = typename decltype(rtag)::type;
                                  using S = typename decltype(stag)::type;
                                  using F = typename decltype(ftag)::type;
                                  constexpr int O = decltype(otag)::value;
-                                 return bdf_launch_one<R, O, S, F>(d, stream, ws_in, ws_bytes);
+                                 return bdf_launch_one<R, O, S, F>(d, stream, ws_in, ws_bytes,
+                                                                   cap_per_cu);
                              });
     if (rc != BG_ERR_UNSUPPORTED) return rc;
-    return bdf_sys_launch_ensemble(method, rhs, semantics, lu_fp32, d, stream, ws_in, ws_bytes);
+    return bdf_sys_launch_ensemble(method, rhs, semantics, lu_fp32, d, stream, ws_in, ws_bytes,
+                                   cap_per_cu);
 }
 
 int bdf_launch_inverse(uint32_t dim, uint64_t count, const double *a, double *inv,

@@ -45,23 +45,23 @@ bool bdf_sys_supported(int method, int rhs, uint32_t dim) {
 }
 
 int bdf_sys_workspace_bytes(int method, int rhs, int semantics, int, const bg_ensemble_desc &d,
-                            size_t *bytes) {
+                            size_t *bytes, int *occupancy) {
     return visit_bdf(method, rhs, semantics, d.dim, [&](auto rtag, auto otag, auto stag) -> int {
         using R = typename decltype(rtag)::type;
         using S = typename decltype(stag)::type;
         constexpr int O = decltype(otag)::value;
         uint64_t grid = 0;
-        return bdf_launch_shape<R, O, S, double>(d, grid, *bytes);
+        return bdf_launch_shape<R, O, S, double>(d, grid, *bytes, 0, occupancy);
     });
 }
 
 int bdf_sys_launch_ensemble(int method, int rhs, int semantics, int, const bg_ensemble_desc &d,
-                            hipStream_t stream, void *ws_in, size_t ws_bytes) {
+                            hipStream_t stream, void *ws_in, size_t ws_bytes, uint32_t cap_per_cu) {
     return visit_bdf(method, rhs, semantics, d.dim, [&](auto rtag, auto otag, auto stag) -> int {
         using R = typename decltype(rtag)::type;
         using S = typename decltype(stag)::type;
         constexpr int O = decltype(otag)::value;
-        return bdf_launch_one<R, O, S, double>(d, stream, ws_in, ws_bytes);
+        return bdf_launch_one<R, O, S, double>(d, stream, ws_in, ws_bytes, cap_per_cu);
     });
 }
 

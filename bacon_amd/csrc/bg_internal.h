@@ -52,6 +52,12 @@ int rk_schedule(const bg_tuning &tn, const bg_ensemble_desc &d, int32_t *order,
 int rk_solve_tuned(int method, int rhs, int semantics, const bg_ensemble_desc &d,
                    const bg_tuning &tn, hipStream_t stream);
 int schedule_by_counts(const bg_counts *counts, uint64_t n, int32_t *order, hipStream_t stream);
+// BDF: the same fitted schedule on the work counters, and the resident
+// waves per CU (tn.launch.blocks_per_cu; blocks are single waves)
+int bdf_autotune(int method, int rhs, int semantics, int lu_fp32, const bg_ensemble_desc &d,
+                 bg_tuning &tn, hipStream_t stream);
+int bdf_solve_tuned(int method, int rhs, int semantics, int lu_fp32, const bg_ensemble_desc &d,
+                    const bg_tuning &tn, hipStream_t stream);
 
 // RK45 for y' = A y + b with one shared A (dim 128), the stage right-hand
 // sides as FP64 MFMA GEMMs over tiles of 16 instances (linear_kernels.hip).
@@ -90,11 +96,14 @@ struct BdfPathState {
 };
 bool bdf_supported(int method, int rhs, uint32_t dim);
 // ws == nullptr: the launcher allocates the workspace; else ws holds
-// ws_bytes, at least bdf_workspace_bytes() for the same arguments.
+// ws_bytes, at least bdf_workspace_bytes() for the same arguments, which
+// also gives the kernel's resident waves (blocks) per CU in *occupancy.
+// cap_per_cu > 0 caps the resident waves per CU.
 int bdf_launch_ensemble(int method, int rhs, int semantics, int lu_fp32, const bg_ensemble_desc &d,
-                        hipStream_t stream, void *ws = nullptr, size_t ws_bytes = 0);
+                        hipStream_t stream, void *ws = nullptr, size_t ws_bytes = 0,
+                        uint32_t cap_per_cu = 0);
 int bdf_workspace_bytes(int method, int rhs, int semantics, int lu_fp32, const bg_ensemble_desc &d,
-                        size_t *bytes);
+                        size_t *bytes, int *occupancy = nullptr);
 int bdf_launch_path(int method, int rhs, int semantics, uint32_t dim, BdfPathState *state,
                     double *path_t, double *path_y, uint32_t cap, hipStream_t stream);
 // the corrector's Jacobian inverse cascade on caller matrices
@@ -103,10 +112,10 @@ int bdf_launch_inverse(uint32_t dim, uint64_t count, const double *a, double *in
 // the small systems (bdf_systems.hip), reached through the functions above
 bool bdf_sys_supported(int method, int rhs, uint32_t dim);
 int bdf_sys_workspace_bytes(int method, int rhs, int semantics, int lu_fp32,
-                            const bg_ensemble_desc &d, size_t *bytes);
+                            const bg_ensemble_desc &d, size_t *bytes, int *occupancy);
 int bdf_sys_launch_ensemble(int method, int rhs, int semantics, int lu_fp32,
                             const bg_ensemble_desc &d, hipStream_t stream, void *ws_in,
-                            size_t ws_bytes);
+                            size_t ws_bytes, uint32_t cap_per_cu);
 int bdf_sys_launch_path(int method, int rhs, int semantics, uint32_t dim, BdfPathState *state,
                         double *path_t, double *path_y, uint32_t cap, hipStream_t stream);
 

@@ -20,6 +20,13 @@
 // Results of an RK solve are bit-identical for every schedule and shape (a
 // lane's arithmetic does not depend on them), so every trial writes a valid
 // solution into the caller's outputs.
+//
+// BDF solves get the same treatment (bdf_autotune): the cost target is the
+// corrector's work, log(16 x Jacobian inverses + Broyden iterations + 1)
+// (an inverse costs about sixteen iterations' instructions on the 32-wide
+// layout), and the launch candidates are caps on the resident waves per CU
+// (one instance per wave, or per lane group for small systems).  A BDF
+// group's arithmetic is its own, so every trial is bit-identical too.
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <cmath>
@@ -61,6 +68,20 @@ __global__ void tune_sample_kernel(const bg_ensemble_desc d, const bg_counts *__
     instance_features(d, i, P, r);
     const double att = (double)counts[i].n_accepted + (double)counts[i].n_rejected;
     r[P + 2] = log(att + 1.0);
+}
+
+// The same rows from a BDF solve's work counters (.n_accepted = Broyden
+// iterations, .n_rejected = Jacobian inverses).
+__global__ void tune_sample_bdf_kernel(const bg_ensemble_desc d, const bg_counts *__restrict__ work,
+                                       uint64_t stride, uint32_t m, uint32_t P,
+                                       double *__restrict__ rows) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    const uint64_t i = (uint64_t)j * stride;
+    double *r = rows + (uint64_t)j * (P + 3);
+    instance_features(d, i, P, r);
+    const double cost = 16.0 * (double)work[i].n_rejected + (double)work[i].n_accepted;
+    r[P + 2] = log(cost + 1.0);
 }
 
 // Sort keys: predicted log cost (descending sort = longest first).
@@ -330,6 +351,120 @@ int schedule_by_counts(const bg_counts *counts, uint64_t n, int32_t *order, hipS
     BG_HIP_CHECK(hipFreeAsync(ws, stream));
     BG_HIP_CHECK(e);
     return BG_OK;
+}
+
+static int timed_bdf(int method, int rhs, int semantics, int lu_fp32, const bg_ensemble_desc &d,
+                     uint32_t cap, hipStream_t stream, double &ms) {
+    hipEvent_t e0, e1;
+    BG_HIP_CHECK(hipEventCreate(&e0));
+    BG_HIP_CHECK(hipEventCreate(&e1));
+    BG_HIP_CHECK(hipEventRecord(e0, stream));
+    int rc = bdf_launch_ensemble(method, rhs, semantics, lu_fp32, d, stream, nullptr, 0, cap);
+    if (rc == BG_OK) {
+        BG_HIP_CHECK(hipEventRecord(e1, stream));
+        BG_HIP_CHECK(hipEventSynchronize(e1));
+        float f = 0;
+        BG_HIP_CHECK(hipEventElapsedTime(&f, e0, e1));
+        ms = f;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
+}
+
+int bdf_autotune(int method, int rhs, int semantics, int lu_fp32, const bg_ensemble_desc &din,
+                 bg_tuning &tn, hipStream_t stream) {
+    memset(&tn, 0, sizeof tn);
+    tn.launch.block = 64;
+    if (!bdf_supported(method, rhs, din.dim)) return BG_ERR_UNSUPPORTED;
+    const uint64_t n = din.n;
+    const uint32_t P = din.n_params < BG_TUNE_MAX_PARAMS ? din.n_params : BG_TUNE_MAX_PARAMS;
+    tn.n_params = P;
+    if (n < 2) return BG_OK;
+    if (n > 0x7fffffffull || !din.y0 || (P && !din.params)) return BG_ERR_INVALID_ARGUMENT;
+    if (din.order) return BG_ERR_INVALID_ARGUMENT;  // the tuner owns the schedule
+#define BG_TRY(expr)                  \
+    if ((expr) != hipSuccess) {       \
+        rc = BG_ERR_HIP;              \
+        break;                        \
+    }
+    bg_ensemble_desc d = din;
+    bg_counts *scratch_work = nullptr;
+    if (!d.work) {
+        BG_HIP_CHECK(hipMallocAsync((void **)&scratch_work, n * sizeof(bg_counts), stream));
+        d.work = scratch_work;
+    }
+    int32_t *order = nullptr;
+    double *rows_d = nullptr;
+    int rc = BG_OK;
+    do {
+        // 1. a solve in input order at the occupancy limit gives the work
+        double ms0 = 0;
+        if ((rc = timed_bdf(method, rhs, semantics, lu_fp32, d, 0, stream, ms0)) != BG_OK) break;
+        tn.pilot_ms = ms0;
+        // 2. the cost model on a deterministic subsample
+        const uint32_t m = (uint32_t)(n < 16384 ? n : 16384);
+        const uint64_t stride = n / m;
+        const uint32_t W = P + 3;
+        BG_TRY(hipMallocAsync((void **)&rows_d, (size_t)m * W * sizeof(double), stream));
+        tune_sample_bdf_kernel<<<(m + 255) / 256, 256, 0, stream>>>(d, d.work, stride, m, P, rows_d);
+        BG_TRY(hipGetLastError());
+        std::vector<double> rows((size_t)m * W);
+        BG_TRY(hipMemcpyAsync(rows.data(), rows_d, rows.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, stream));
+        BG_TRY(hipStreamSynchronize(stream));
+        tn.pilot_n = m;
+        fit_model(rows, m, P, tn);
+        // 3. schedule, then the resident-wave caps
+        BG_TRY(hipMallocAsync((void **)&order, n * sizeof(int32_t), stream));
+        if ((rc = schedule(tn, d, order, stream)) != BG_OK) break;
+        d.order = order;
+        // caps of 4, 3, 2 and 1 waves per SIMD below the occupancy limit
+        size_t ws_bytes = 0;
+        int occ = 0;
+        if ((rc = bdf_workspace_bytes(method, rhs, semantics, lu_fp32, d, &ws_bytes, &occ)) !=
+            BG_OK)
+            break;
+        double best = 1e300;
+        for (const uint32_t cap : {0u, 16u, 12u, 8u, 4u}) {
+            if (cap && (int)cap >= occ) continue;
+            double ms = 0;
+            if ((rc = timed_bdf(method, rhs, semantics, lu_fp32, d, cap, stream, ms)) != BG_OK)
+                break;
+            ++tn.n_trials;
+            if (cap == 0) tn.default_ms = ms;
+            if (ms < best) {
+                best = ms;
+                tn.launch.blocks_per_cu = cap;
+            }
+        }
+        if (rc != BG_OK) break;
+        tn.best_ms = best;
+    } while (0);
+#undef BG_TRY
+    if (rows_d) hipFreeAsync(rows_d, stream);
+    if (order) hipFreeAsync(order, stream);
+    if (scratch_work) hipFreeAsync(scratch_work, stream);
+    const hipError_t e = hipStreamSynchronize(stream);
+    if (rc == BG_OK && e != hipSuccess) rc = BG_ERR_HIP;
+    return rc;
+}
+
+int bdf_solve_tuned(int method, int rhs, int semantics, int lu_fp32, const bg_ensemble_desc &din,
+                    const bg_tuning &tn, hipStream_t stream) {
+    const uint32_t cap = tn.launch.blocks_per_cu;
+    if (din.order || din.n < 2)
+        return bdf_launch_ensemble(method, rhs, semantics, lu_fp32, din, stream, nullptr, 0, cap);
+    bg_ensemble_desc d = din;
+    int32_t *order = nullptr;
+    BG_HIP_CHECK(hipMallocAsync((void **)&order, d.n * sizeof(int32_t), stream));
+    int rc = schedule(tn, d, order, stream);
+    if (rc == BG_OK) {
+        d.order = order;
+        rc = bdf_launch_ensemble(method, rhs, semantics, lu_fp32, d, stream, nullptr, 0, cap);
+    }
+    BG_HIP_CHECK(hipFreeAsync(order, stream));
+    return rc;
 }
 
 int rk_solve_tuned(int method, int rhs, int semantics, const bg_ensemble_desc &din,

@@ -761,6 +761,28 @@ def main():
             else:
                 entry["max_abs_diff_vs_fp64_lu"] = float(np.abs(yb - y64).max())
             bdf[f"lu_fp{bits}"] = entry
+        # the tuner's plan (fitted longest-first schedule + resident-wave cap)
+        tn = L.Tuning()
+        rc = cx.lib.bg_bdf_autotune_device(L.BDF6, L.RHS["brusselator"], L.INTENDED, 64,
+                                           C.byref(db), C.byref(tn), cx.sh)
+        if rc != 0:
+            raise RuntimeError(rc)
+
+        def run_bt():
+            rc = cx.lib.bg_bdf_solve_tuned_device(L.BDF6, L.RHS["brusselator"], L.INTENDED, 64,
+                                                  C.byref(db), C.byref(tn), cx.sh)
+            if rc != 0:
+                raise RuntimeError(rc)
+        wall_t, _ = time_launches(cx, run_bt, 1, 1)
+        bdf["tuned_lu_fp64"] = {
+            "plan": {"waves_per_cu_cap": int(tn.launch.blocks_per_cu) or "occupancy",
+                     "schedule": "longest-first by the fitted cost model",
+                     "coef": [float(c) for c in tn.coef[:6]]},
+            "default_plan": {"waves_per_cu_cap": "occupancy", "schedule": "input order"},
+            "tuner_ms": {"pilot_input_order": tn.pilot_ms, "fitted_schedule_occupancy":
+                         tn.default_ms, "best": tn.best_ms, "trials": int(tn.n_trials)},
+            "ms_per_solve": wall_t * 1e3, "solves_per_s": nb_ / wall_t,
+            "bit_identical_to_untuned": bool(np.array_equal(bb["y_out"].cpu().numpy(), y64))}
         result["bdf6_brusselator"] = bdf
         del bb, db
 

@@ -322,6 +322,20 @@ void bg_graph_destroy(bg_graph *g);
 int bg_bdf_solve_device(int method, int rhs, int semantics, int lu_bits,
                         const bg_ensemble_desc *d, void *stream);
 
+/* The autotuner for a BDF workload (no counterpart in the reference): the
+ * schedule model of bg_tuning fitted to the corrector's work per instance
+ * (16 x Jacobian inverses + Broyden iterations; d->work is filled if set),
+ * then the cap on resident waves per CU (tuning->launch.blocks_per_cu, 0 =
+ * occupancy; launch.block is 64) timed on the whole problem with that
+ * schedule.  d->order must be NULL.  Runs 1 + n_trials full solves, each
+ * writing valid results into d's outputs; returns when they are done. */
+int bg_bdf_autotune_device(int method, int rhs, int semantics, int lu_bits,
+                           const bg_ensemble_desc *d, bg_tuning *out, void *stream);
+/* Schedule + solve with a BDF plan, asynchronous (d->order, if set, is used
+ * as given). */
+int bg_bdf_solve_tuned_device(int method, int rhs, int semantics, int lu_bits,
+                              const bg_ensemble_desc *d, const bg_tuning *tuning, void *stream);
+
 /* The BDF corrector's Jacobian inverse (bdf.rs:429-444) on `count` caller
  * matrices a[count][dim][dim] (row-major, device memory), dim in {1, 2, 4,
  * 8, 16, 32}: partial-pivot Gauss-Jordan, then full-pivot Gauss-Jordan,
