@@ -311,7 +311,7 @@ struct AdamsLane {
 
 template <class RHS, int O, bool AsBuilt>
 __global__ void __launch_bounds__(256)
-    adams_ensemble_kernel(const volatile bg_ensemble_desc *dv, uint32_t *__restrict__ counter) {
+    adams_ensemble_kernel(const bg_ensemble_desc *dv, uint32_t *__restrict__ counter) {
     constexpr int D = RHS::D;
     AdamsLane<RHS, O, AsBuilt> L;
     const int lane = __lane_id();
@@ -330,10 +330,10 @@ __global__ void __launch_bounds__(256)
             if (inst == -1) {
                 const uint64_t pos =
                     (uint64_t)base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
-                if (pos < dv->n) {
-                    const int32_t *order = dv->order;
-                    inst = order ? (int64_t)order[pos] : (int64_t)pos;
-                    L.load(desc_inputs(dv), (uint64_t)inst);
+                const bg_ensemble_desc di = desc_inputs(dv);
+                if (pos < di.n) {
+                    inst = di.order ? (int64_t)di.order[pos] : (int64_t)pos;
+                    L.load(di, (uint64_t)inst);
                     steps = nacc = 0;
                 } else {
                     inst = -2;

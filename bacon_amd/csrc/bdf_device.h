@@ -1059,7 +1059,7 @@ struct BdfLane {
 // measured 20 % faster than the 2 that fit without spilling (4: slower).
 template <class RHS, int O, class Sem, class LuT, int SPLIT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
-    bdf_ensemble_kernel(const volatile bg_ensemble_desc *dv, uint32_t *__restrict__ counter) {
+    bdf_ensemble_kernel(const bg_ensemble_desc *dv, uint32_t *__restrict__ counter) {
     constexpr int W = RHS::W;
     constexpr int GW = W * SPLIT;  // lanes per instance
     constexpr int GPW = 64 / GW;   // groups per wave
@@ -1093,10 +1093,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
             const bool my_need = (need >> (grp_id * GW)) & 1ull;
             if (my_need) {
                 const uint64_t k = (uint64_t)base + (uint64_t)__popcll(below);
-                if (k < dv->n) {
-                    const int32_t *order = dv->order;
-                    inst = order ? (int64_t)order[k] : (int64_t)k;
-                    L.load(desc_inputs(dv), (uint64_t)inst);
+                const bg_ensemble_desc di = desc_inputs(dv);
+                if (k < di.n) {
+                    inst = di.order ? (int64_t)di.order[k] : (int64_t)k;
+                    L.load(di, (uint64_t)inst);
                 } else {
                     inst = -2;
                 }

@@ -397,12 +397,21 @@ struct RkLane {
     }
 };
 
-// Descriptor reads through a volatile pointer to a device-memory copy: each
-// field is a fresh load at the point of use, so the persistent kernel does not
-// keep ~30 descriptor SGPRs live across the attempt loop.
-__device__ __forceinline__ bg_ensemble_desc desc_inputs(const volatile bg_ensemble_desc *dv) {
+// The descriptor's device-memory copy, read at the point of use.  The empty
+// asm hides the pointer's value from the optimiser, so the loads are not
+// hoisted out of the persistent loop (no ~30 descriptor registers live across
+// the attempts) while staying ordinary cached loads that issue together under
+// one wait.  (Volatile reads would be uncached, each waited for in turn.)
+__device__ __forceinline__ const bg_ensemble_desc *desc_at_use(const bg_ensemble_desc *dv) {
+    asm volatile("" : "+s"(dv));
+    return dv;
+}
+
+__device__ __forceinline__ bg_ensemble_desc desc_inputs(const bg_ensemble_desc *dv_) {
+    const bg_ensemble_desc *dv = desc_at_use(dv_);
     bg_ensemble_desc d;
     d.n = dv->n;
+    d.order = dv->order;
     d.y0 = dv->y0;
     d.params = dv->params;
     d.t0 = dv->t0;
@@ -418,7 +427,8 @@ __device__ __forceinline__ bg_ensemble_desc desc_inputs(const volatile bg_ensemb
     return d;
 }
 
-__device__ __forceinline__ bg_ensemble_desc desc_outputs(const volatile bg_ensemble_desc *dv) {
+__device__ __forceinline__ bg_ensemble_desc desc_outputs(const bg_ensemble_desc *dv_) {
+    const bg_ensemble_desc *dv = desc_at_use(dv_);
     bg_ensemble_desc d;
     d.n = dv->n;
     d.t_out = dv->t_out;

@@ -95,7 +95,7 @@ __device__ __forceinline__ int rk_advance(RkLane<RHS, Tab> &L, uint32_t &attempt
 // raised wave priority so the longest solves are not starved.
 template <class RHS, class Tab>
 __global__ void __launch_bounds__(256)
-    rk_ensemble_kernel(const volatile bg_ensemble_desc *dv, uint32_t *__restrict__ counter,
+    rk_ensemble_kernel(const bg_ensemble_desc *dv, uint32_t *__restrict__ counter,
                        uint32_t refill_min, uint32_t prio_cutoff) {
     r8_init();
     RkLane<RHS, Tab> L;
@@ -117,11 +117,10 @@ __global__ void __launch_bounds__(256)
             if (inst == kNeedWork) {
                 const uint64_t below = need & ((1ull << lane) - 1ull);
                 pos = (uint64_t)base + (uint64_t)__popcll(below);
-                const uint64_t n = dv->n;
-                if (pos < n) {
-                    const int32_t *order = dv->order;
-                    inst = order ? (int64_t)order[pos] : (int64_t)pos;
-                    rk_load(L, desc_inputs(dv), (uint64_t)inst);
+                const bg_ensemble_desc di = desc_inputs(dv);  // one batch of loads
+                if (pos < di.n) {
+                    inst = di.order ? (int64_t)di.order[pos] : (int64_t)pos;
+                    rk_load(L, di, (uint64_t)inst);
                     attempts = nacc = nrej = 0;
                 } else {
                     inst = kExhausted;
@@ -159,7 +158,7 @@ __global__ void __launch_bounds__(256)
 // stop time and an accepted clamped step lands exactly on it.
 template <class RHS, class Tab>
 __global__ void __launch_bounds__(256)
-    rk_path_ensemble_kernel(const volatile bg_ensemble_desc *dv, const bg_path_desc *__restrict__ pd,
+    rk_path_ensemble_kernel(const bg_ensemble_desc *dv, const bg_path_desc *__restrict__ pd,
                             uint32_t *__restrict__ counter) {
     constexpr int D = RHS::D;
     r8_init();
@@ -194,11 +193,10 @@ __global__ void __launch_bounds__(256)
             if (inst == kNeedWork) {
                 const uint64_t pos =
                     (uint64_t)base + (uint64_t)__popcll(need & ((1ull << lane) - 1ull));
-                const uint64_t n = dv->n;
-                if (pos < n) {
-                    const int32_t *order = dv->order;
-                    inst = order ? (int64_t)order[pos] : (int64_t)pos;
-                    rk_load(L, desc_inputs(dv), (uint64_t)inst);
+                const bg_ensemble_desc di = desc_inputs(dv);
+                if (pos < di.n) {
+                    inst = di.order ? (int64_t)di.order[pos] : (int64_t)pos;
+                    rk_load(L, di, (uint64_t)inst);
                     attempts = nacc = nrej = next_stop = nrec = 0;
                     every_left = every;
                     slot = 0;
@@ -237,7 +235,7 @@ __global__ void __launch_bounds__(256)
                         const bool rec = stops_only ? (at_stop || last) : (every_left == 0);
                         if (every_left == 0) every_left = every;
                         if (rec) {
-                            const uint64_t n = dv->n, i = (uint64_t)inst;
+                            const uint64_t n = desc_at_use(dv)->n, i = (uint64_t)inst;
                             if (capacity) {
                                 const uint64_t e = slot;
                                 if (inst_major) {
