@@ -98,6 +98,15 @@ class AdjointDesc(C.Structure):
     ]
 
 
+class AdjointTuning(C.Structure):
+    _fields_ = [("checkpoint_every", C.c_uint32), ("max_checkpoints", C.c_uint32),
+                ("blocks_per_cu", C.c_uint32), ("n_trials", C.c_uint32),
+                ("default_ms", C.c_double), ("best_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # (name, restype, argtypes) of every exported entry point
 SIGNATURES = [
     ("bg_new", C.c_int, [C.c_int, C.c_int, C.c_uint32, C.POINTER(C.c_void_p)]),
@@ -173,6 +182,8 @@ SIGNATURES = [
      [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.c_void_p]),
     ("bg_rk_vjp_device", C.c_int,
      [C.c_int, C.c_int, C.POINTER(AdjointDesc), C.c_void_p]),
+    ("bg_adjoint_autotune_device", C.c_int,
+     [C.c_int, C.c_int, C.POINTER(AdjointDesc), C.POINTER(AdjointTuning), C.c_void_p]),
     ("bg_version", C.c_int, []),
     ("bg_device_info", C.c_int,
      [C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),

@@ -39,18 +39,11 @@ class Schedule:
         self.order = longest_first(counts, stream)
 
 
-def solve_vjp(rhs: str, y0: torch.Tensor, params: torch.Tensor, obs_t: torch.Tensor,
-              targets: torch.Tensor, t_end, tol, dt_min: float = 1e-12, dt_max: float = 0.1,
-              t0: float = 0.0, checkpoint_every: int = 16, max_checkpoints: int = 128,
-              stream=None, order=None, blocks_per_cu: int = 0, schedule: Schedule = None):
-    """All tensors float64 on the GPU: y0 [D, n], params [P, n], obs_t [n_obs],
-    targets [n_obs, D, n]; t_end / tol scalars or [n] tensors.  `order` (int32
-    [n], optional) is the forward pass's schedule, e.g. from
-    bg_rk_schedule_device with a plan tuned on the same problem.  With a
-    `Schedule` instead, the solve uses the order it carries (if any) and
-    leaves it the order for the next solve."""
-    if schedule is not None and order is None:
-        order = schedule.current(y0.shape[1])
+def _vjp_call(rhs, y0, params, obs_t, targets, t_end, tol, dt_min, dt_max, t0,
+              checkpoint_every, max_checkpoints, stream, order, blocks_per_cu, tuning=None):
+    """Builds the adjoint descriptor and output tensors, then runs
+    bg_rk_vjp_device, or bg_adjoint_autotune_device when `tuning` (an
+    L.AdjointTuning) is given."""
     dev = y0.device
     D, n = y0.shape
     P = params.shape[0]
@@ -80,12 +73,54 @@ def solve_vjp(rhs: str, y0: torch.Tensor, params: torch.Tensor, obs_t: torch.Ten
                       obs_y=_ptr(out["obs_y"]))
     sh = C.c_void_p(stream.cuda_stream if stream is not None
                     else torch.cuda.current_stream(dev).cuda_stream)
-    rc = L.lib().bg_rk_vjp_device(L.RK45, L.RHS[rhs], C.byref(d), sh)
+    if tuning is None:
+        rc = L.lib().bg_rk_vjp_device(L.RK45, L.RHS[rhs], C.byref(d), sh)
+        name = "bg_rk_vjp_device"
+    else:
+        rc = L.lib().bg_adjoint_autotune_device(L.RK45, L.RHS[rhs], C.byref(d),
+                                                C.byref(tuning), sh)
+        name = "bg_adjoint_autotune_device"
     if rc != 0:
-        raise RuntimeError(f"bg_rk_vjp_device returned {rc}")
+        raise RuntimeError(f"{name} returned {rc}")
+    return out
+
+
+def solve_vjp(rhs: str, y0: torch.Tensor, params: torch.Tensor, obs_t: torch.Tensor,
+              targets: torch.Tensor, t_end, tol, dt_min: float = 1e-12, dt_max: float = 0.1,
+              t0: float = 0.0, checkpoint_every: int = 16, max_checkpoints: int = 128,
+              stream=None, order=None, blocks_per_cu: int = 0, schedule: Schedule = None,
+              plan: "L.AdjointTuning" = None):
+    """All tensors float64 on the GPU: y0 [D, n], params [P, n], obs_t [n_obs],
+    targets [n_obs, D, n]; t_end / tol scalars or [n] tensors.  `order` (int32
+    [n], optional) is the forward pass's schedule, e.g. from
+    bg_rk_schedule_device with a plan tuned on the same problem.  With a
+    `Schedule` instead, the solve uses the order it carries (if any) and
+    leaves it the order for the next solve.  `plan` (from `autotune_vjp`)
+    replaces checkpoint_every, max_checkpoints and blocks_per_cu."""
+    if schedule is not None and order is None:
+        order = schedule.current(y0.shape[1])
+    if plan is not None:
+        checkpoint_every, max_checkpoints = plan.checkpoint_every, plan.max_checkpoints
+        blocks_per_cu = plan.blocks_per_cu
+    out = _vjp_call(rhs, y0, params, obs_t, targets, t_end, tol, dt_min, dt_max, t0,
+                    checkpoint_every, max_checkpoints, stream, order, blocks_per_cu)
     if schedule is not None:
         schedule.update(out["counts"], stream)
     return out
+
+
+def autotune_vjp(rhs: str, y0: torch.Tensor, params: torch.Tensor, obs_t: torch.Tensor,
+                 targets: torch.Tensor, t_end, tol, dt_min: float = 1e-12, dt_max: float = 0.1,
+                 t0: float = 0.0, checkpoint_every: int = 16, max_checkpoints: int = 128,
+                 stream=None, order=None, blocks_per_cu: int = 0) -> "L.AdjointTuning":
+    """The adjoint's plan (bg_adjoint_autotune_device): the fastest checkpoint
+    interval and resident blocks per CU for this problem, timed on the whole
+    gradient solve; the arguments are solve_vjp's.  Returns L.AdjointTuning
+    (pass it to solve_vjp as `plan`)."""
+    tn = L.AdjointTuning()
+    _vjp_call(rhs, y0, params, obs_t, targets, t_end, tol, dt_min, dt_max, t0,
+              checkpoint_every, max_checkpoints, stream, order, blocks_per_cu, tuning=tn)
+    return tn
 
 
 def longest_first(counts: torch.Tensor, stream=None) -> torch.Tensor:

@@ -873,6 +873,12 @@ extern "C" int bg_rk_vjp_device(int method, int rhs, const bg_adjoint_desc *d, v
     return rk_launch_adjoint(method, rhs, *d, (hipStream_t)stream);
 }
 
+extern "C" int bg_adjoint_autotune_device(int method, int rhs, const bg_adjoint_desc *d,
+                                          bg_adjoint_tuning *out, void *stream) {
+    if (!d || !out || method != BG_RK45) return BG_ERR_INVALID_ARGUMENT;
+    return adjoint_autotune(method, rhs, *d, *out, (hipStream_t)stream);
+}
+
 extern "C" int bg_version(void) { return 1; }
 
 extern "C" int bg_device_info(int *n_cu, int *arch_major, int *arch_minor) {

@@ -119,8 +119,10 @@ int bdf_sys_launch_ensemble(int method, int rhs, int semantics, int lu_fp32,
 int bdf_sys_launch_path(int method, int rhs, int semantics, uint32_t dim, BdfPathState *state,
                         double *path_t, double *path_y, uint32_t cap, hipStream_t stream);
 
-// adjoint (adjoint_kernels.hip)
+// adjoint (adjoint_kernels.hip), and its tuner (tune.hip)
 int rk_launch_adjoint(int method, int rhs, const bg_adjoint_desc &d, hipStream_t stream);
+int adjoint_autotune(int method, int rhs, const bg_adjoint_desc &d, bg_adjoint_tuning &out,
+                     hipStream_t stream);
 
 int device_cu_count(int device);
 

@@ -21,6 +21,10 @@
 // lane's arithmetic does not depend on them), so every trial writes a valid
 // solution into the caller's outputs.
 //
+// The adjoint's tuner (adjoint_autotune) times the gradient solve over
+// checkpoint intervals and resident blocks per CU; gradients are the same
+// bits for all of them.
+//
 // BDF solves get the same treatment (bdf_autotune): the cost target is the
 // corrector's work, log(16 x Jacobian inverses + Broyden iterations + 1)
 // (an inverse costs about sixteen iterations' instructions on the 32-wide
@@ -464,6 +468,60 @@ int bdf_solve_tuned(int method, int rhs, int semantics, int lu_fp32, const bg_en
         rc = bdf_launch_ensemble(method, rhs, semantics, lu_fp32, d, stream, nullptr, 0, cap);
     }
     BG_HIP_CHECK(hipFreeAsync(order, stream));
+    return rc;
+}
+
+int adjoint_autotune(int method, int rhs, const bg_adjoint_desc &din, bg_adjoint_tuning &tn,
+                     hipStream_t stream) {
+    memset(&tn, 0, sizeof tn);
+    const uint32_t K0 = din.checkpoint_every ? din.checkpoint_every : 16u;
+    const uint32_t M0 = din.max_checkpoints ? din.max_checkpoints : 128u;
+    const uint64_t steps = (uint64_t)K0 * M0;  // accepted steps the caller's plan covers
+    struct Trial {
+        uint32_t K, bpc;
+    };
+    Trial trials[12];
+    int nt = 0;
+    for (const uint32_t K : {K0, 16u, 32u, 48u})
+        for (const uint32_t bpc : {din.blocks_per_cu, 0u, 1u, 2u}) {
+            bool seen = false;
+            for (int q = 0; q < nt; ++q) seen |= trials[q].K == K && trials[q].bpc == bpc;
+            if (!seen && nt < 12) trials[nt++] = Trial{K, bpc};
+        }
+    hipEvent_t e0, e1;
+    BG_HIP_CHECK(hipEventCreate(&e0));
+    BG_HIP_CHECK(hipEventCreate(&e1));
+    int rc = BG_OK;
+    double best = 1e300;
+    for (int q = 0; q < nt && rc == BG_OK; ++q) {
+        bg_adjoint_desc d = din;
+        d.checkpoint_every = trials[q].K;
+        d.max_checkpoints = (uint32_t)((steps + trials[q].K - 1) / trials[q].K);
+        d.blocks_per_cu = trials[q].bpc;
+        if (hipEventRecord(e0, stream) != hipSuccess) {
+            rc = BG_ERR_HIP;
+            break;
+        }
+        rc = rk_launch_adjoint(method, rhs, d, stream);
+        if (rc != BG_OK) break;
+        float ms = 0;
+        if (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = BG_ERR_HIP;
+            break;
+        }
+        ++tn.n_trials;
+        if (q == 0) tn.default_ms = ms;  // the caller's own K and blocks_per_cu
+        if (ms < best) {
+            best = ms;
+            tn.checkpoint_every = d.checkpoint_every;
+            tn.max_checkpoints = d.max_checkpoints;
+            tn.blocks_per_cu = d.blocks_per_cu;
+        }
+    }
+    tn.best_ms = tn.n_trials ? best : 0.0;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
     return rc;
 }
 

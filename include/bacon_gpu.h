@@ -389,6 +389,26 @@ typedef struct {
  * instance) on `stream`, asynchronous. */
 int bg_rk_vjp_device(int method, int rhs, const bg_adjoint_desc *d, void *stream);
 
+/* A tuned plan for the adjoint (no counterpart in the reference): the
+ * checkpoint interval K and the resident blocks per CU.  Gradients do not
+ * depend on either (each segment is re-run from its checkpoint with the
+ * forward's operations), so the plan only changes the time. */
+typedef struct {
+    uint32_t checkpoint_every;  /* K of the fastest trial                       */
+    uint32_t max_checkpoints;   /* scaled so that K x max_checkpoints covers the
+                                   caller's accepted steps per instance        */
+    uint32_t blocks_per_cu;     /* 0 = the launcher's defaults                 */
+    uint32_t n_trials;
+    double default_ms;          /* the caller's K and blocks_per_cu            */
+    double best_ms;
+} bg_adjoint_tuning;
+/* Times the gradient solve of `d` for K in {d's K, 16, 32, 48} and
+ * blocks_per_cu in {0, 1, 2} (distinct pairs), each trial writing valid
+ * results into d's outputs; returns when they are done.  Apply the plan by
+ * copying its three fields into the descriptor. */
+int bg_adjoint_autotune_device(int method, int rhs, const bg_adjoint_desc *d,
+                               bg_adjoint_tuning *out, void *stream);
+
 /* Library / device information */
 int bg_version(void);
 int bg_device_info(int *n_cu, int *arch_major, int *arch_minor);

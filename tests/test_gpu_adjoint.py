@@ -173,6 +173,27 @@ def test_tape_in_lds_and_hbm_agree(gpu, lv):
             assert np.array_equal(g[key], base[key]), (k, key)
 
 
+def test_autotune_plan_bit_identical(gpu, lv):
+    """bg_adjoint_autotune_device (bacon_amd.adjoint.autotune_vjp) times the
+    gradient solve over K in {16, 32, 48} and 0-2 blocks per CU: 9 distinct
+    trials from the default K = 16, each writing valid results.  Its plan
+    keeps K x max_checkpoints at the caller's 16 x 128 steps, and the solve
+    with the plan returns the default solve's bits."""
+    from bacon_amd.adjoint import autotune_vjp
+    base = _gpu(lv)
+    tn = autotune_vjp("lotka_volterra", _t(lv["y0"]), _t(lv["params"]), _t(lv["obs"]),
+                      _t(lv["targets"]), 15.0, TOL)
+    print("adjoint plan:", tn.as_dict())
+    assert tn.n_trials == 9
+    assert tn.checkpoint_every in (16, 32, 48) and tn.blocks_per_cu in (0, 1, 2)
+    assert tn.max_checkpoints == -(-16 * 128 // tn.checkpoint_every)
+    assert 0.0 < tn.best_ms <= tn.default_ms
+    g = _gpu(lv, plan=tn)
+    assert np.all(g["status"] == 0)
+    for key in ("grad_params", "grad_y0", "loss", "obs_y", "counts"):
+        assert np.array_equal(g[key], base[key]), key
+
+
 def test_adjoint_in_measured_order_bit_identical(gpu, lv):
     """The forward pass scheduled longest first by an earlier solve's counts
     (bacon_amd.adjoint.longest_first) changes no bits."""
