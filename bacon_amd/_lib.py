@@ -29,7 +29,7 @@ DIM_STATIC, DIM_DYN = 0, 1
 
 RHS = {
     "exp": 0, "neg": 1, "quadratic": 2, "cos": 3, "lorenz": 4, "linear": 5,
-    "brusselator": 6, "lotka_volterra": 7, "oscillator": 8, "fail_after": 9, "robertson": 10,
+    "brusselator": 6, "lotka_volterra": 7, "oscillator": 8, "fail_after": 9, "robertson": 10, "two_level": 11,
 }
 
 _dp = C.POINTER(C.c_double)

@@ -517,6 +517,7 @@ static int visit_adams(int method, int rhs, int semantics, uint32_t dim, V &&v) 
     case BG_RHS_LOTKA_VOLTERRA: return with(AdTag<RhsLotkaVolterra>{});
     case BG_RHS_OSCILLATOR: return with(AdTag<RhsOscillator>{});
     case BG_RHS_ROBERTSON: return with(AdTag<RhsRobertson>{});
+    case BG_RHS_TWO_LEVEL: return with(AdTag<RhsTwoLevel>{});
     default: return BG_ERR_UNSUPPORTED;
     }
 }

@@ -43,6 +43,7 @@ static int rhs_param_count(int rhs) {
     case BG_RHS_LORENZ: return 3;
     case BG_RHS_BRUSSELATOR: return 3;
     case BG_RHS_ROBERTSON: return 3;
+    case BG_RHS_TWO_LEVEL: return 3;
     case BG_RHS_LOTKA_VOLTERRA: return 4;
     case BG_RHS_FAIL_AFTER: return 1;
     case BG_RHS_EXP: case BG_RHS_NEG: case BG_RHS_QUADRATIC: case BG_RHS_COS:

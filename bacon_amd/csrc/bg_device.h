@@ -223,6 +223,29 @@ struct RhsRobertson {
     }
 };
 
+// Complex state (bacon-sci's N: ComplexField): a two-level system
+// c' = (-i H - gamma) c, H = [[hd, ho], [ho, -hd]], params hd, ho, gamma.
+// The two complex components are stored as (re, im) pairs, y = (a0, b0, a1,
+// b1) with c_k = a_k + i b_k: -i H c = H b - i H a.  With real times and
+// step sizes the solver's complex products by real coefficients act on the
+// parts separately, so the pairs run as four reals; |c|^2 decays as
+// exp(-2 gamma t).
+struct RhsTwoLevel {
+    static constexpr int D = 4, P = 3;
+    __device__ static bool eval(double, const double (&y)[4], const double *p, double (&f)[4]) {
+        const double hd = p[0], ho = p[1], g = p[2];
+        const double ha0 = fma(hd, y[0], ho * y[2]);
+        const double ha1 = fma(ho, y[0], -(hd * y[2]));
+        const double hb0 = fma(hd, y[1], ho * y[3]);
+        const double hb1 = fma(ho, y[1], -(hd * y[3]));
+        f[0] = fma(-g, y[0], hb0);
+        f[1] = fma(-g, y[1], -ha0);
+        f[2] = fma(-g, y[2], hb1);
+        f[3] = fma(-g, y[3], -ha1);
+        return true;
+    }
+};
+
 struct RhsOscillator {
     static constexpr int D = 2, P = 0;
     __device__ static bool eval(double, const double (&y)[2], const double *, double (&f)[2]) {

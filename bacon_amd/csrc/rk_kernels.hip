@@ -394,6 +394,8 @@ static int visit_rk(int method, int rhs, int semantics, uint32_t dim, V &&v) {
         return dim == 2 ? with_tab(Tag<RhsOscillator>{}) : BG_ERR_INVALID_ARGUMENT;
     case BG_RHS_ROBERTSON:
         return dim == 3 ? with_tab(Tag<RhsRobertson>{}) : BG_ERR_INVALID_ARGUMENT;
+    case BG_RHS_TWO_LEVEL:
+        return dim == 4 ? with_tab(Tag<RhsTwoLevel>{}) : BG_ERR_INVALID_ARGUMENT;
     default:
         return BG_ERR_UNSUPPORTED;
     }

@@ -133,6 +133,34 @@ def robertson(n: int = 4096, first: int = 0):
                 dt_max=5e-4)
 
 
+def two_level(n: int = 4096, first: int = 0):
+    """Complex two-level systems c' = (-i H - gamma) c, H = [[hd, ho], [ho,
+    -hd]]: hd, ho ~ U[-2, 2], gamma ~ logU[0.01, 0.2]; c0 a random unit
+    vector of C^2 stored as (re, im) pairs (dim 4); t in [0, 10], tol 1e-8,
+    dt in [1e-12, 0.1] (workload index 6)."""
+    s = Stream(6, n, first)
+    params = np.stack([s.uniform(-2.0, 2.0), s.uniform(-2.0, 2.0), s.log_uniform(0.01, 0.2)])
+    y0 = np.stack([s.normal() for _ in range(4)])
+    y0 /= np.sqrt((y0 ** 2).sum(0))
+    return dict(rhs="two_level", dim=4, y0=y0, params=params, t0=0.0,
+                t_end=np.full(n, 10.0), tol=np.full(n, 1e-8), dt_min=1e-12, dt_max=0.1)
+
+
+def two_level_exact(params, y0, t):
+    """c(t) = exp(-gamma t) (cos(w t) I - i sin(w t) H / w) c0, w = |(hd, ho)|,
+    per instance (params [3, n], y0 [4, n] as (re, im) pairs); returns [4, n]."""
+    hd, ho, g = params
+    c = np.stack([y0[0] + 1j * y0[1], y0[2] + 1j * y0[3]])
+    w = np.hypot(hd, ho)
+    co, si = np.cos(w * t), np.sin(w * t) / w
+    h0 = hd * c[0] + ho * c[1]
+    h1 = ho * c[0] - hd * c[1]
+    d = np.exp(-g * t)
+    r0 = d * (co * c[0] - 1j * si * h0)
+    r1 = d * (co * c[1] - 1j * si * h1)
+    return np.stack([r0.real, r0.imag, r1.real, r1.imag])
+
+
 def linear_shared(n: int = 65536, first: int = 0, N: int = 128):
     """Linear y' = A y + b with one shared A = -(Q D Q^T): Q the sign-fixed
     orthogonal factor of a seeded N(0,1) matrix, D log-uniform in [0.1,10],

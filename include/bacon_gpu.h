@@ -85,6 +85,13 @@ extern "C" {
 #define BG_RHS_OSCILLATOR 8     /* y0' = y1, y1' = -y0 (ivp.rs cos_deriv)    */
 #define BG_RHS_FAIL_AFTER 9     /* y' = y; UserError once t >= params[0]     */
 #define BG_RHS_ROBERTSON 10     /* stiff kinetics; params: k1, k2, k3; dim 3 */
+/* Complex state (the reference's N: ComplexField, rk.rs:53, adams.rs:44): a
+ * system of m complex components is 2m reals, each component stored as its
+ * (re, im) pair; RK and Adams only (BDF's Broyden update multiplies by the
+ * plain transpose, bdf.rs:461-466, so its complex run is not the real-pair
+ * run).  Two-level system c' = (-i H - gamma) c, H = [[hd, ho], [ho, -hd]]:
+ * params hd, ho, gamma; dim 4 (two complex components). */
+#define BG_RHS_TWO_LEVEL 11
 
 /* ======================================================================== */
 /* Single-instance builder + iterator                                       */

@@ -43,7 +43,7 @@ PROFILE_BACON, PROFILE_DEVICE = 0, 1
 RHS = {
     "exp": 0, "neg": 1, "quadratic": 2, "cos": 3, "lorenz": 4, "linear": 5,
     "brusselator": 6, "lotka_volterra": 7, "oscillator": 8, "fail_after": 9,
-    "lv_sens": 10, "robertson": 11,
+    "lv_sens": 10, "robertson": 11, "two_level": 12,
 }
 
 _dp = C.POINTER(C.c_double)
@@ -129,6 +129,9 @@ def lib():
         L.or_rootn.restype = C.c_double
         L.or_root8.argtypes = [C.c_double]
         L.or_root8.restype = C.c_double
+        L.or_norm.argtypes = [C.c_int32, _dp, C.c_int32, C.c_int32]
+        L.or_norm.restype = C.c_double
+        L.or_rhs_is_complex.argtypes = [C.c_int32]
         L.or_rk_replay.argtypes = [C.POINTER(Problem), C.c_int64, _dp, _dp, _dp]
         for name in ("or_builder_new", "or_builder_new_dyn"):
             getattr(L, name).argtypes = [C.POINTER(Builder), C.c_int32, C.c_int32]
@@ -179,8 +182,11 @@ def solve(method=RK45, rhs="lorenz", y0=(1.0,), t0=0.0, t_end=1.0, tol=1e-6,
         pdt = np.zeros(path_cap)
         path = Path(cap=path_cap, len=0, t=_ptr(pt), y=_ptr(py), dt=_ptr(pdt))
     stop_y = None if stops is None else np.zeros((stops.size, dim))
-    lib().or_solve(C.byref(p), _ptr(y_out), C.byref(r),
-                   C.byref(path) if path is not None else None, _ptr(stop_y))
+    rc = lib().or_solve(C.byref(p), _ptr(y_out), C.byref(r),
+                        C.byref(path) if path is not None else None, _ptr(stop_y))
+    if rc == -1:
+        raise ValueError("or_solve: invalid argument (method, dimension, stops or "
+                         "right-hand side not supported together)")
     out = dict(t=r.t, dt=r.dt, y=y_out, status=r.status, n_accepted=r.n_accepted,
                n_rejected=r.n_rejected, n_rhs=r.n_rhs, n_newton=r.n_newton,
                n_lu=r.n_lu, n_yield=r.n_yield, stop_y=stop_y)
@@ -284,6 +290,13 @@ def rootn(x, n):
 def root8(r):
     """The DEVICE profile's step-factor root r^(1/8), before the tangent bound."""
     return lib().or_root8(float(r))
+
+
+def norm(x, profile=PROFILE_BACON, complex_pairs=False):
+    """The error norm of a step: nalgebra's norm() (BACON; over complex
+    (re, im) pairs with complex_pairs) or the device's fused sum (DEVICE)."""
+    x = _f64(x)
+    return lib().or_norm(x.size, _ptr(x), profile, int(complex_pairs))
 
 
 def rk_replay(rhs, y0, t_seq, dt_seq, params=None, method=RK45,

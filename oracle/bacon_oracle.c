@@ -72,6 +72,34 @@ static double nal_norm(int n, const double *x) {
     return sqrt(res);
 }
 
+/* nalgebra's norm() for a column vector of nc complex entries stored as
+ * (re, im) pairs: dotc(x, x) sums conj(x_i) x_i, whose real part is
+ * re_i re_i - (-im_i) im_i, one term per complex entry, with dotc's
+ * accumulation order over the entries (sequential below 8, eight
+ * interleaved accumulators above). */
+static double nal_norm_c(int nc, const double *x) {
+    double res = 0.0;
+    double acc[8];
+    int i = 0;
+    for (int k = 0; k < 8; ++k) acc[k] = 0.0;
+    while (nc - i >= 8) {
+        for (int k = 0; k < 8; ++k) {
+            const double re = x[2 * (i + k)], im = x[2 * (i + k) + 1];
+            acc[k] += re * re - (-im) * im;
+        }
+        i += 8;
+    }
+    res += acc[0] + acc[4];
+    res += acc[1] + acc[5];
+    res += acc[2] + acc[6];
+    res += acc[3] + acc[7];
+    for (; i < nc; ++i) {
+        const double re = x[2 * i], im = x[2 * i + 1];
+        res += re * re - (-im) * im;
+    }
+    return sqrt(res);
+}
+
 /* Device-order sum of squares: sequential fused accumulation. */
 static double dev_sumsq(int n, const double *x) {
     double s = x[0] * x[0];
@@ -106,6 +134,19 @@ static double dev_root8(double r) {
 /* ------------------------------------------------------------------ */
 /* right-hand sides                                                    */
 /* ------------------------------------------------------------------ */
+
+/* Complex state (bacon-sci's N: ComplexField, rk.rs:53, adams.rs:44): with
+ * real times and step sizes, the solvers' complex products by real
+ * coefficients act on the real and imaginary parts separately, so a system
+ * of m complex components runs as 2m reals stored (re, im); only the error
+ * norm groups each pair (nal_norm_c). */
+int32_t or_rhs_is_complex(int32_t rhs) { return rhs == OR_RHS_TWO_LEVEL; }
+
+double or_norm(int32_t n, const double *x, int32_t profile, int32_t complex_pairs) {
+    if (n < 1) return 0.0;
+    if (profile == OR_PROFILE_DEVICE) return sqrt(dev_sumsq(n, x));
+    return complex_pairs ? nal_norm_c(n / 2, x) : nal_norm(n, x);
+}
 
 int32_t or_rhs_eval(int32_t rhs, int32_t prof, int32_t dim, double t,
                     const double *y, const double *pr, const double *shared,
@@ -191,6 +232,33 @@ int32_t or_rhs_eval(int32_t rhs, int32_t prof, int32_t dim, double t,
         f[0] = b - a;
         f[1] = (a - b) - c;
         f[2] = c;
+        return OR_OK;
+    }
+    case OR_RHS_TWO_LEVEL: {
+        /* y = (a0, b0, a1, b1), components c_k = a_k + i b_k;
+         * -i H c = H b - i H a; then -gamma c */
+        if (dim != 4) return OR_INVALID_ARGUMENT;
+        const double hd = pr[0], ho = pr[1], g = pr[2];
+        double ha0, ha1, hb0, hb1;
+        if (prof == OR_PROFILE_DEVICE) {
+            ha0 = fma(hd, y[0], ho * y[2]);
+            ha1 = fma(ho, y[0], -(hd * y[2]));
+            hb0 = fma(hd, y[1], ho * y[3]);
+            hb1 = fma(ho, y[1], -(hd * y[3]));
+            f[0] = fma(-g, y[0], hb0);
+            f[1] = fma(-g, y[1], -ha0);
+            f[2] = fma(-g, y[2], hb1);
+            f[3] = fma(-g, y[3], -ha1);
+        } else {
+            ha0 = hd * y[0] + ho * y[2];
+            ha1 = ho * y[0] - hd * y[2];
+            hb0 = hd * y[1] + ho * y[3];
+            hb1 = ho * y[1] - hd * y[3];
+            f[0] = hb0 - g * y[0];
+            f[1] = -ha0 - g * y[1];
+            f[2] = hb1 - g * y[2];
+            f[3] = -ha1 - g * y[3];
+        }
         return OR_OK;
     }
     case OR_RHS_FAIL_AFTER:
@@ -449,7 +517,9 @@ static int rk_step(rk_solver *s, double *used_dt) {
         const double f = fmin(f0, tang);
         factor = accepted ? fmax(f, 1.0) : fmin(f, 0x1.fffffffffffffp-1);
     } else {
-        const double error = nal_norm(D, scratch) / s->dt;
+        const double nrm = or_rhs_is_complex(p->rhs) ? nal_norm_c(D / 2, scratch)
+                                                      : nal_norm(D, scratch);
+        const double error = nrm / s->dt;
         accepted = error <= s->tol;
         const double delta =
             s->point_eighty_four * pow(s->tol / error, s->one_fourth);
@@ -1002,6 +1072,10 @@ static int bdf_secant(bdf_solver *s, int lower, double *guess) {
 static int bdf_init(bdf_solver *s, const or_problem *p) {
     memset(s, 0, offsetof(bdf_solver, jinv));
     s->p = p;
+    /* Complex state is left out of BDF: the Broyden update (bdf.rs:461-466)
+     * multiplies by shift.transpose(), not its conjugate transpose, so a
+     * complex system does not run as its real-pair system there. */
+    if (or_rhs_is_complex(p->rhs)) return OR_INVALID_ARGUMENT;
     s->O = or_bdf_coefficients(p->method, s->higher, s->lower);
     if (s->O < 0) return OR_INVALID_ARGUMENT;
     s->D = p->dim;
@@ -1376,7 +1450,9 @@ static int adams_step(adams_solver *s, double *yt, double *yv) {
     }
     for (int d = 0; d < D; ++d) corrector[d] = madd(pr, scratch[d], s->dt, s->state[d]);
     for (int d = 0; d < D; ++d) diff[d] = corrector[d] - predictor[d];
-    const double nrm = pr == OR_PROFILE_DEVICE ? sqrt(dev_sumsq(D, diff)) : nal_norm(D, diff);
+    const double nrm = pr == OR_PROFILE_DEVICE  ? sqrt(dev_sumsq(D, diff))
+                       : or_rhs_is_complex(s->p->rhs) ? nal_norm_c(D / 2, diff)
+                                                      : nal_norm(D, diff);
     const double error = s->err_coef / s->dt * nrm;
     if (error <= s->tol) { /* adams.rs:490-525 */
         memcpy(s->state, corrector, sizeof(double) * (size_t)D);

@@ -86,7 +86,10 @@ enum {
     OR_RHS_FAIL_AFTER = 9,   /* y' = y, UserError once t >= params[0]         */
     OR_RHS_LV_SENS = 10,     /* 14-dim forward-sensitivity system of LV       */
     OR_RHS_ROBERTSON = 11,   /* stiff kinetics, params k1, k2, k3; dim 3      */
-    OR_RHS_COUNT = 12
+    OR_RHS_TWO_LEVEL = 12,   /* complex N: y' = (-i H - gamma) y, H = [[hd, ho],
+                                [ho, -hd]], params hd, ho, gamma; two complex
+                                components stored (re, im) pairs: dim 4       */
+    OR_RHS_COUNT = 13
 };
 
 #define OR_MAX_DIM 128
@@ -138,6 +141,15 @@ int32_t or_solve_ensemble(const or_problem *tmpl, int64_t n, int32_t n_params,
                           int32_t threads);
 
 /* Evaluate one RHS (for tests of the device functors). */
+/* Norm of an error vector as the given profile's step computes it: BACON is
+ * nalgebra's norm() (dotc, eight accumulators from 8 rows), over n reals or,
+ * with complex_pairs, over n/2 complex components stored as (re, im) pairs
+ * (each component contributes re^2 + im^2 as one term); DEVICE is the
+ * sequential fused sum of squares over the n reals. */
+double or_norm(int32_t n, const double *x, int32_t profile, int32_t complex_pairs);
+/* 1 for a right-hand side with complex state (OR_RHS_TWO_LEVEL). */
+int32_t or_rhs_is_complex(int32_t rhs);
+
 int32_t or_rhs_eval(int32_t rhs, int32_t profile, int32_t dim, double t,
                     const double *y, const double *params,
                     const double *shared, double *out);

@@ -625,3 +625,70 @@ def test_adams_lorenz_against_golden(orc):
             assert all(a > b for a, b in zip(errs, errs[1:])), (method, i, errs)
             if method == O.ADAMS5:
                 assert errs[-1] < 1e-9, (i, errs)
+
+
+# ---- complex state (N: ComplexField, rk.rs:53, adams.rs:44) ------------------
+
+def test_complex_norm_groups_each_component():
+    """nalgebra's dotc over complex entries adds re^2 + im^2 as one term per
+    entry.  With three entries (1, 0), (a, a), (a, a), a^2 = 0.72 half-ulps
+    of 1: summing the six reals one at a time never leaves 1, while each
+    complex term (1.44 half-ulps) rounds up, so only the grouped norm exceeds
+    1.  The device profile is the sequential fused sum of the reals."""
+    a = 1.2 * 2.0 ** -27
+    x = np.array([1.0, 0.0, a, a, a, a])
+    assert O.norm(x) == 1.0
+    assert O.norm(x, profile=O.PROFILE_DEVICE) == 1.0
+    assert O.norm(x, complex_pairs=True) == np.nextafter(1.0, 2.0)
+    # away from rounding ties all three agree with numpy's complex norm
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        z = rng.standard_normal(4) + 1j * rng.standard_normal(4)
+        xr = np.stack([z.real, z.imag], 1).ravel()
+        ref = np.linalg.norm(z)
+        for kw in ({}, {"complex_pairs": True}, {"profile": O.PROFILE_DEVICE}):
+            assert abs(O.norm(xr, **kw) - ref) <= 4e-16 * ref
+
+
+def test_two_level_rhs_is_the_complex_product():
+    """The registered two-level RHS equals (-i H - gamma) c computed in
+    complex arithmetic, in both profiles, to rounding."""
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        hd, ho = rng.uniform(-2, 2, 2)
+        g = rng.uniform(0.01, 0.2)
+        y = rng.standard_normal(4)
+        c = np.array([y[0] + 1j * y[1], y[2] + 1j * y[3]])
+        H = np.array([[hd, ho], [ho, -hd]])
+        f = -1j * (H @ c) - g * c
+        want = np.array([f[0].real, f[0].imag, f[1].real, f[1].imag])
+        for prof in (O.PROFILE_BACON, O.PROFILE_DEVICE):
+            rc, got = O.rhs_eval("two_level", y, params=[hd, ho, g], profile=prof)
+            assert rc == O.OK
+            assert np.allclose(got, want, rtol=1e-14, atol=1e-14 * np.abs(want).max())
+
+
+def test_two_level_against_closed_form():
+    """RK45, RK23 and Adams5 (both profiles) on 32 complex two-level systems
+    against c(t) = e^{-gamma t} exp(-iHt) c0 at t = 10, tol 1e-8: within
+    2e-6 (20 tol t_end), and the squared modulus decays as e^{-2 gamma t}."""
+    w = W.two_level(32)
+    ex = W.two_level_exact(w["params"], w["y0"], 10.0)
+    for method in (O.RK45, O.RK23, O.ADAMS5):
+        for prof in (O.PROFILE_BACON, O.PROFILE_DEVICE):
+            for i in range(32):
+                r = O.solve(method, "two_level", w["y0"][:, i], 0.0, 10.0, 1e-8, 1e-12, 0.1,
+                            params=w["params"][:, i], semantics=O.INTENDED, profile=prof)
+                assert r["status"] == O.OK and r["t"] == 10.0
+                assert np.abs(r["y"] - ex[:, i]).max() < 2e-6, (method, prof, i)
+                decay = np.exp(-2 * w["params"][2, i] * 10.0)
+                assert abs((r["y"] ** 2).sum() - decay) < 2e-6
+
+
+def test_bdf_rejects_complex_state():
+    """BDF's Broyden update uses the plain transpose (bdf.rs:461-466), so a
+    complex system is not its real-pair system there: the oracle refuses."""
+    w = W.two_level(1)
+    with pytest.raises(ValueError):
+        O.solve(O.BDF6, "two_level", w["y0"][:, 0], 0.0, 1.0, 1e-8, 1e-12, 0.1,
+                params=w["params"][:, 0], semantics=O.INTENDED)
