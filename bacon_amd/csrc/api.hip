@@ -613,7 +613,7 @@ extern "C" int bg_ensemble_solve_plain(bg_ensemble *e) {
 // that does not support the right-hand side is skipped.
 static bool method_supports(int method, int rhs, uint32_t dim) {
     if (is_adams(method)) return adams_supported(method, rhs, dim);
-    if (is_rk(method)) return rhs == BG_RHS_LINEAR ? dim == 128 && method == BG_RK45
+    if (is_rk(method)) return rhs == BG_RHS_LINEAR ? linear_supported(dim) && method == BG_RK45
                                                    : rk_supported(method, rhs, dim);
     return bdf_supported(method, rhs, dim);
 }

@@ -59,8 +59,10 @@ int bdf_autotune(int method, int rhs, int semantics, int lu_fp32, const bg_ensem
 int bdf_solve_tuned(int method, int rhs, int semantics, int lu_fp32, const bg_ensemble_desc &d,
                     const bg_tuning &tn, hipStream_t stream);
 
-// RK45 for y' = A y + b with one shared A (dim 128), the stage right-hand
-// sides as FP64 MFMA GEMMs over tiles of 16 instances (linear_kernels.hip).
+// RK45 for y' = A y + b with one shared A (dim 32, 64 or 128), the stage
+// right-hand sides as FP64 MFMA GEMMs over tiles of 16 instances
+// (linear_kernels.hip).
+bool linear_supported(uint32_t dim);
 int linear_launch_ensemble(int method, int semantics, const bg_ensemble_desc &d,
                            hipStream_t stream, void *ws = nullptr);
 

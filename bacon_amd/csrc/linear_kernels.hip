@@ -1,15 +1,16 @@
 // linear_kernels.hip — batched RK45 for y' = A y + b with one shared dense A
-// (N = 128), the stage right-hand side as an FP64 MFMA GEMM.
+// (N = 32, 64 or 128), the stage right-hand side as an FP64 MFMA GEMM.
 //
 // Hot path: bacon-sci's RungeKuttaSolver::step (src/ivp/rk.rs:361-423) for a
 // linear right-hand side shared by every instance.  For a tile of T = 16
-// instances the six stage evaluations are products A * Y with Y the 128 x 16
+// instances the six stage evaluations are products A * Y with Y the N x 16
 // matrix of stage inputs: dense FP64 GEMMs on v_mfma_f64_16x16x4f64.
 //
-// Workgroup = 4 waves = one tile of 16 instances (the MFMA N dimension).
+// Workgroup = N / 32 waves = one tile of 16 instances (the MFMA N dimension).
 //  * Wave w owns rows [32w, 32w+32) of A as MFMA A-operands, resident in
-//    registers for the whole launch (2 M-tiles x 32 k-steps = 64 doubles per
-//    lane): A is read from HBM once per workgroup.
+//    registers for the whole launch (2 M-tiles x N/4 k-steps: 64 doubles per
+//    lane at N = 128): A is read from HBM once per workgroup.  Beyond N = 128
+//    A no longer fits the registers of a workgroup's waves.
 //  * v_mfma_f64_16x16x4f64 operand layout on gfx950 (D = A B + C, 16x4 by
 //    4x16; checked with a probe kernel): lane l supplies A[l % 16][l / 16]
 //    and B[l / 16][l % 16], and register r of lane l holds
@@ -18,10 +19,10 @@
 //    32w + 16 mt + 4 r + l/16 (mt = 0,1; r = 0..3) of y, of the six stage
 //    vectors K_i and of the stage input — exactly the MFMA accumulator layout,
 //    so the RK combinations are per-lane FMAs on the GEMM outputs.
-//  * Each stage writes its input Y into a 128 x 16 LDS tile (the B operands
-//    of all four waves), barrier, 2 x 32 MFMAs per wave, barrier.
+//  * Each stage writes its input Y into an N x 16 LDS tile (the B operands
+//    of all the waves), barrier, 2 x N/4 MFMAs per wave, barrier.
 //  * The error norm of each instance reduces over its 8 rows per lane, over
-//    the 4 lanes of the same column (shuffles), and over the 4 waves (LDS) in
+//    the 4 lanes of the same column (shuffles), and over the waves (LDS) in
 //    a fixed order, so every lane holding the column computes the same
 //    controller decision.  The controller is RkLane's (squared-norm form).
 //  * Instances are taken per column from a global counter: a column whose
@@ -36,16 +37,16 @@ namespace bg {
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
-static constexpr int LN = 128;      // dimension
 static constexpr int LT = 16;       // instances per tile
-static constexpr int LWAVES = 4;    // waves per workgroup
-static constexpr int LRW = LN / LWAVES;  // rows per wave (32)
+static constexpr int LRW = 32;      // rows per wave (two 16-row M-tiles)
 static constexpr uint32_t kLinMaxAttempts = 10000000u;
 
-template <class Tab>
-__global__ void __launch_bounds__(256)
+// LWAVES waves per workgroup, dimension LN = 32 LWAVES.
+template <class Tab, int LWAVES>
+__global__ void __launch_bounds__(64 * LWAVES)
     linear_rk_kernel(const bg_ensemble_desc *__restrict__ dp, uint32_t *__restrict__ counter) {
     constexpr int S = Tab::S;
+    constexpr int LN = LRW * LWAVES;
     const bg_ensemble_desc &d = *dp;
     __shared__ double ytile[LN][LT];
     __shared__ double red[LWAVES][LT];
@@ -234,7 +235,9 @@ __global__ void __launch_bounds__(256)
             ss += __shfl_xor(ss, 32);
             if (q == 0) red[w][col] = ss;
             __syncthreads();
-            ss = ((red[0][col] + red[1][col]) + red[2][col]) + red[3][col];
+            ss = red[0][col];
+#pragma unroll
+            for (int v = 1; v < LWAVES; ++v) ss += red[v][col];
             __syncthreads();
             // ---- controller (rk.rs:392-416), squared form as RkLane::attempt_to
             if (step) {
@@ -276,6 +279,11 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+template <class T>
+struct LinTab {
+    using type = T;
+};
+
 __global__ void linear_setup_kernel(uint32_t *counter, bg_ensemble_desc *dst,
                                     const bg_ensemble_desc d) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -284,17 +292,20 @@ __global__ void linear_setup_kernel(uint32_t *counter, bg_ensemble_desc *dst,
     }
 }
 
+bool linear_supported(uint32_t dim) { return dim == 32 || dim == 64 || dim == 128; }
+
 int linear_launch_ensemble(int method, int semantics, const bg_ensemble_desc &d,
                            hipStream_t stream, void *ws_in) {
     if (method != BG_RK45) return BG_ERR_UNSUPPORTED;
-    if (d.dim != (uint32_t)LN) return BG_ERR_UNSUPPORTED;
+    if (!linear_supported(d.dim)) return BG_ERR_UNSUPPORTED;
     if (d.n == 0) return BG_OK;
     if (d.n > 0x7fffffffull || !d.y0 || !d.shared) return BG_ERR_INVALID_ARGUMENT;
+    const int threads = 2 * (int)d.dim;  // 64 per 32 rows
     auto run = [&](auto *kern) -> int {
         int dev = 0;
         BG_HIP_CHECK(hipGetDevice(&dev));
         int per_cu = 0;
-        BG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0));
+        BG_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, 0));
         if (per_cu < 1) per_cu = 1;
         uint64_t grid = (uint64_t)per_cu * (uint64_t)device_cu_count(dev);
         const uint64_t tiles = (d.n + LT - 1) / LT;
@@ -304,14 +315,20 @@ int linear_launch_ensemble(int method, int semantics, const bg_ensemble_desc &d,
         uint32_t *counter = (uint32_t *)ws;
         bg_ensemble_desc *ddesc = (bg_ensemble_desc *)(ws + 256);
         linear_setup_kernel<<<1, 64, 0, stream>>>(counter, ddesc, d);
-        kern<<<(uint32_t)grid, 256, 0, stream>>>(ddesc, counter);
+        kern<<<(uint32_t)grid, threads, 0, stream>>>(ddesc, counter);
         const hipError_t launched = hipGetLastError();
         if (!ws_in) BG_HIP_CHECK(hipFreeAsync(ws, stream));
         BG_HIP_CHECK(launched);
         return BG_OK;
     };
-    if (semantics == BG_AS_BUILT) return run(linear_rk_kernel<Tableau<Fehlberg45, true>>);
-    if (semantics == BG_INTENDED) return run(linear_rk_kernel<Tableau<Fehlberg45, false>>);
+    auto by_dim = [&](auto tab) -> int {
+        using T = typename decltype(tab)::type;
+        if (d.dim == 32) return run(linear_rk_kernel<T, 1>);
+        if (d.dim == 64) return run(linear_rk_kernel<T, 2>);
+        return run(linear_rk_kernel<T, 4>);
+    };
+    if (semantics == BG_AS_BUILT) return by_dim(LinTab<Tableau<Fehlberg45, true>>{});
+    if (semantics == BG_INTENDED) return by_dim(LinTab<Tableau<Fehlberg45, false>>{});
     return BG_ERR_INVALID_ARGUMENT;
 }
 

@@ -79,7 +79,8 @@ extern "C" {
 #define BG_RHS_QUADRATIC 2      /* y' = -2t (rk.rs:664-666)                  */
 #define BG_RHS_COS 3            /* y' = cos t (rk.rs:668-670)                */
 #define BG_RHS_LORENZ 4         /* params: sigma, rho, beta; dim 3           */
-#define BG_RHS_LINEAR 5         /* y' = A y + b, shared [A row-major; b]     */
+#define BG_RHS_LINEAR 5         /* y' = A y + b, shared [A row-major; b];
+                                   dim 32, 64 or 128; RK45                   */
 #define BG_RHS_BRUSSELATOR 6    /* params: A, B, alpha; dim 2*cells          */
 #define BG_RHS_LOTKA_VOLTERRA 7 /* params: alpha, beta, gamma, delta; dim 2  */
 #define BG_RHS_OSCILLATOR 8     /* y0' = y1, y1' = -y0 (ivp.rs cos_deriv)    */

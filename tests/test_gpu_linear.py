@@ -1,5 +1,6 @@
-"""GPU RK45 on y' = A y + b with one shared 128 x 128 matrix (the FP64 MFMA
-tile kernel) against the oracle and the closed-form solution.
+"""GPU RK45 on y' = A y + b with one shared N x N matrix (the FP64 MFMA
+tile kernel; N = 128, and 32 and 64) against the oracle and the closed-form
+solution.
 
 The GEMM sums A y in a different order than the oracle's loop, so the two
 agree to rounding per evaluation and their accepted-step sequences may part
@@ -30,10 +31,12 @@ def _oracle(w, i, t_end, tol, semantics=O.INTENDED, dt_min=1e-12, dt_max=0.1,
                    max_attempts=max_attempts)
 
 
-def test_linear_vs_oracle_and_closed_form(gpu, orc):
+@pytest.mark.parametrize("N", [128, 64, 32])
+def test_linear_vs_oracle_and_closed_form(gpu, orc, N):
     """37 instances (two full tiles of 16 and a ragged one), tol 1e-9 on
-    [0, 5]: |y_gpu - y_exact| and |y_gpu - y_oracle| <= 2 tol max(1, |y|)."""
-    w = W.linear_shared(37)
+    [0, 5], for N = 128 (4 waves per tile), 64 (2) and 32 (1):
+    |y_gpu - y_exact| and |y_gpu - y_oracle| <= 2 tol max(1, |y|)."""
+    w = W.linear_shared(37, N=N)
     g = ivp.solve_ensemble(L.RK45, w)
     assert np.all(g["status"] == 0)
     assert np.all(g["t"] == 5.0)
@@ -48,13 +51,14 @@ def test_linear_vs_oracle_and_closed_form(gpu, orc):
         assert abs(int(g["n_accepted"][i]) - r["n_accepted"]) <= 0.02 * r["n_accepted"] + 1
 
 
-def test_linear_vs_bacon_profile(gpu, orc):
+@pytest.mark.parametrize("N", [128, 32])
+def test_linear_vs_bacon_profile(gpu, orc, N):
     """64 instances of the shared-matrix workload (tol 1e-9 on [0, 5])
     against the reference's own arithmetic (the oracle's BACON profile:
     separate multiply and add in the stage sums and in A y, the norm-based
     controller with pow): within 10x tol everywhere."""
     n = 64
-    w = W.linear_shared(n)
+    w = W.linear_shared(n, N=N)
     g = ivp.solve_ensemble(L.RK45, w)
     assert np.all(g["status"] == 0)
     worst = 0.0
@@ -150,7 +154,7 @@ def test_linear_minimum_dt_and_attempt_cap(gpu, orc):
 
 def test_linear_device_entry_and_unsupported(gpu):
     """bg_rk_solve_device on a tile-ragged batch matches the builder; the
-    linear right-hand side needs dim 128 and RK45."""
+    linear right-hand side needs dim 32, 64 or 128 and RK45."""
     import ctypes as C
 
     import torch
@@ -176,6 +180,6 @@ def test_linear_device_entry_and_unsupported(gpu):
     assert np.array_equal(y_out.cpu().numpy(), g["y"])
     assert L.lib().bg_rk_solve_device(L.RK23, L.RHS["linear"], L.INTENDED, C.byref(d),
                                       None, None) == -3
-    d.dim = 64
+    d.dim = 96
     assert L.lib().bg_rk_solve_device(L.RK45, L.RHS["linear"], L.INTENDED, C.byref(d),
                                       None, None) == -3
