@@ -1054,11 +1054,13 @@ struct BdfLane {
 // ---------------------------------------------------------------------------
 // persistent ensemble kernel: one instance per group, group refill
 // ---------------------------------------------------------------------------
-// At least 3 waves per SIMD: the solver is latency-bound (LDS and
-// cross-lane exchanges on every step), and 3 waves with a few spill slots
-// measured 20 % faster than the 2 that fit without spilling (4: slower).
+// Four waves per SIMD (the LDS limit): the solver is latency-bound (an LDS
+// round trip on every elimination step), and waves hide each other's
+// waits.  On the 32-component Brusselator, 4 waves with 101 VGPRs spilled
+// measured 392 ms against 395 ms for 3 waves with 43 spilled (FP32 inverse:
+// 240 against 248 ms); 3 had beaten 2 by 20 % before.
 template <class RHS, int O, class Sem, class LuT, int SPLIT>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
     bdf_ensemble_kernel(const bg_ensemble_desc *dv, uint32_t *__restrict__ counter) {
     constexpr int W = RHS::W;
     constexpr int GW = W * SPLIT;  // lanes per instance
