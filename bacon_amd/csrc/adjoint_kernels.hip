@@ -11,14 +11,17 @@
 // like the ensemble kernel:
 //  * forward: the adaptive solve, with the end clamp of rk.rs:366-368 aimed at
 //    the next observation time (the state lands exactly on it).  Every K
-//    accepted steps it writes a checkpoint (t, dt, next stop, y) to HBM; at
-//    each observation it stores the residual y(t_k) - target_k.
-//  * backward: segments newest first.  A segment re-runs the adaptive solve
-//    from its checkpoint (bit-identical to the forward: same operations),
-//    recording its K accepted steps (t, dt, stop flag, y) on a per-lane tape
-//    in LDS, then sweeps them in reverse: lambda += 2 r_k at an observation, and the
-//    stage-by-stage VJP of the step map with the right-hand side's
-//    hand-written Jacobian-vector products.
+//    accepted steps it writes a checkpoint (t, dt, next stop, y) to HBM, and
+//    it records the size of every accepted step; at each observation it
+//    stores the residual y(t_k) - target_k.
+//  * backward: segments newest first.  A segment re-runs its K accepted steps
+//    from its checkpoint with the recorded step sizes (bit-identical to the
+//    forward: the same stage and update operations; the rejected attempts
+//    and the controller are not repeated), recording (t, dt, stop flag, y)
+//    per step on a per-lane tape in LDS, then sweeps them in reverse:
+//    lambda += 2 r_k at an observation, and the stage-by-stage VJP of the
+//    step map with the right-hand side's hand-written Jacobian-vector
+//    products.
 // Storage layouts are structure-of-arrays with the instance (or lane slot) as
 // the fastest index, so each lane's loads and stores coalesce with its wave.
 #include <rocprim/device/device_radix_sort.hpp>
@@ -32,6 +35,8 @@ static constexpr uint32_t kAdjMaxAttempts = 10000000u;
 
 struct AdjWs {
     double *ck;      // [n][maxck][F], F = 3 + D: t, dt, next_stop, y[D]
+    double *hs;      // [n][maxck][K] size of each accepted step, negated for a
+                     // step that lands on an observation time
     uint32_t *nck;   // [n] checkpoints written
     uint32_t *nobs;  // [n] observations reached
     double *res;     // [n][n_obs][D] residuals y(t_k) - target_k
@@ -146,11 +151,16 @@ __global__ void __launch_bounds__(256)
                 } else {
                     const bool to_stop = next_stop < n_stops;
                     bool clamped;
-                    const int r = L.attempt_to(target, to_stop, clamped);
+                    double h;
+                    const int r = L.attempt_to(target, to_stop, clamped, h);
                     ++attempts;
                     if (r == STEP_ACCEPTED) {
-                        ++nacc;
                         const uint64_t i = (uint64_t)inst;
+                        // accepted step nacc lies in segment nacc / K < maxck
+                        // (the capacity test below ends the solve at the
+                        // segment boundary past the last checkpoint)
+                        ws.hs[i * maxck * K + nacc] = (clamped && to_stop) ? -h : h;
+                        ++nacc;
                         if (clamped && to_stop) {
                             L.t = target;
 #pragma unroll
@@ -329,6 +339,17 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int f = 0; f < F; ++f) ck[f] = ws.ck[(i * maxck + (uint64_t)sg) * F + f];
     };
+    // the first four recorded step sizes of segment `sg`, loaded before they
+    // are needed (at the instance start, and before the previous segment's
+    // reverse sweep); the replay keeps four loads in flight
+    double h0 = 0.0, h1 = 0.0, h2 = 0.0, h3 = 0.0;
+    auto load_h = [&](int sg, uint64_t i) {
+        const double *hs = ws.hs + (i * maxck + (uint64_t)sg) * K;
+        h0 = hs[0];
+        h1 = K > 1 ? hs[1] : 0.0;
+        h2 = K > 2 ? hs[2] : 0.0;
+        h3 = K > 3 ? hs[3] : 0.0;
+    };
     // residual of the stop the reverse sweep meets next (stops are met in
     // decreasing order, n_stops - 1 first), loaded one stop ahead
     int res_k = -1;
@@ -370,7 +391,10 @@ __global__ void __launch_bounds__(256)
                     const int fw = d.status ? gptr(d.status)[i] : BG_OK;
                     seg = fw == BG_OK ? (int)ws.nck[i] - 1 : -1;
                     load_res((int)n_stops - 1, i);
-                    if (seg >= 0) load_ck(seg, i);
+                    if (seg >= 0) {
+                        load_ck(seg, i);
+                        load_h(seg, i);
+                    }
                 } else {
                     inst = -2;
                 }
@@ -385,42 +409,35 @@ __global__ void __launch_bounds__(256)
                     fin = true;
                 } else {
                     const uint64_t i = (uint64_t)inst;
-                    // ---- recompute segment `seg` from its checkpoint ----
+                    // ---- re-run segment `seg` from its checkpoint ----
                     L.t = ck[0];
-                    L.dt = ck[1];
                     uint32_t next_stop = (uint32_t)ck[2];
 #pragma unroll
                     for (int c = 0; c < D; ++c) L.y[c] = ck[3 + c];
                     if (seg > 0) load_ck(seg - 1, i);
                     double target = next_stop < n_stops ? obs[next_stop] : L.end;
-                    uint32_t cnt = 0, guard = 0;
-                    while (cnt < K && L.t < L.end && guard < kAdjMaxAttempts) {
-                        ++guard;
-                        const bool to_stop = next_stop < n_stops;
-                        const double t_before = L.t;
-                        double y_before[D];
+                    const double *hs = ws.hs + (i * maxck + (uint64_t)seg) * K;
+                    uint32_t cnt = 0;
+                    while (cnt < K && L.t < L.end) {
+                        const double hk = h0;  // negative: the step lands on a stop
+                        h0 = h1;
+                        h1 = h2;
+                        h2 = h3;
+                        h3 = cnt + 4 < K ? hs[cnt + 4] : 0.0;
+                        double *rec = tape + ((uint64_t)cnt * FT) * S;
+                        rec[0 * S] = L.t;
+                        rec[1 * S] = hk;
 #pragma unroll
-                        for (int c = 0; c < D; ++c) y_before[c] = L.y[c];
-                        const double dt_used = (L.t + L.dt >= target) ? target - L.t : L.dt;
-                        bool clamped;
-                        const int r = L.attempt_to(target, to_stop, clamped);
-                        if (r == STEP_ACCEPTED) {
-                            const bool at_stop = clamped && to_stop;
-                            if (at_stop) {
-                                L.t = target;
-                                ++next_stop;
-                                target = next_stop < n_stops ? obs[next_stop] : L.end;
-                            }
-                            double *rec = tape + ((uint64_t)cnt * FT) * S;
-                            rec[0 * S] = t_before;
-                            rec[1 * S] = at_stop ? -dt_used : dt_used;
-#pragma unroll
-                            for (int c = 0; c < D; ++c) rec[(2 + c) * S] = y_before[c];
-                            ++cnt;
-                        } else if (r != STEP_REJECTED) {
-                            break;  // cannot happen on a replay of a successful forward
+                        for (int c = 0; c < D; ++c) rec[(2 + c) * S] = L.y[c];
+                        if (!L.advance(fabs(hk))) break;  // not on a replay of a successful forward
+                        if (signbit(hk)) {
+                            L.t = target;
+                            ++next_stop;
+                            target = next_stop < n_stops ? obs[next_stop] : L.end;
                         }
+                        ++cnt;
                     }
+                    if (seg > 0) load_h(seg - 1, i);
                     // ---- reverse sweep ----
                     // Observation steps take the stop indices in reverse:
                     // the replay ended with next_stop one past the last.
@@ -557,7 +574,8 @@ int rk_launch_adjoint(int method, int rhs, const bg_adjoint_desc &ad, hipStream_
             nullptr, sort_bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
             (const int32_t *)nullptr, (int32_t *)nullptr, (uint32_t)n, 0, 32, stream));
         const size_t bytes = 256 + sizeof(bg_adjoint_desc) + 64 +
-                             sizeof(double) * (maxck * F * n + n_res + K * FT * slots) +
+                             sizeof(double) * (maxck * F * n + maxck * K * n + n_res +
+                                               K * FT * slots) +
                              sizeof(uint32_t) * n + 256 + 5 * (sizeof(uint32_t) * n + 256) +
                              sort_bytes;
         char *buf = nullptr;
@@ -568,7 +586,8 @@ int rk_launch_adjoint(int method, int rhs, const bg_adjoint_desc &ad, hipStream_
         char *p = buf + 256 + sizeof(bg_adjoint_desc);
         p += (64 - ((uintptr_t)p % 64)) % 64;
         ws.ck = (double *)p;
-        ws.res = ws.ck + maxck * F * n;
+        ws.hs = ws.ck + maxck * F * n;
+        ws.res = ws.hs + maxck * K * n;
         ws.seg = ws.res + n_res;
         ws.nck = (uint32_t *)(ws.seg + K * FT * slots);
         ws.slots = slots;

@@ -338,13 +338,13 @@ struct RkLane {
     // below dt_min and fail a solve that succeeds without the stop (at the
     // end the solve is done, so the reference clamp is unaffected).
     __device__ __forceinline__ int attempt_to(double target, bool to_stop, bool &clamped) {
-        clamped = false;
-        const double dt_proposed = dt;
-        if (t + dt >= target) {
-            dt = target - t;
-            clamped = true;
-        }
-        // rk.rs:370-384
+        double h;
+        return attempt_to(target, to_stop, clamped, h);
+    }
+
+    // The stages of a step of size h from (t, y): rk.rs:370-384.  False if
+    // the right-hand side reports an error.
+    __device__ __forceinline__ bool stages(double h) {
 #pragma unroll
         for (int i = 0; i < S; ++i) {
             double Y[D];
@@ -358,12 +358,50 @@ struct RkLane {
                     for (int d = 0; d < D; ++d) Y[d] = fma(kc(m), K[j][d], Y[d]);
                 }
             }
-            const double ti = fma(Tab::c(i), dt, t);
+            const double ti = fma(Tab::c(i), h, t);
             double f[D];
-            if (!RHS::eval(ti, Y, p, f)) return BG_USER_ERROR;
+            if (!RHS::eval(ti, Y, p, f)) return false;
 #pragma unroll
-            for (int d = 0; d < D; ++d) K[i][d] = f[d] * dt;
+            for (int d = 0; d < D; ++d) K[i][d] = f[d] * h;
         }
+        return true;
+    }
+
+    // The accepted step's update from the stages: rk.rs:392-398.
+    __device__ __forceinline__ void update(double h) {
+        t += h;
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            const double b = Tab::b(j);
+            if (b != 0.0) {
+#pragma unroll
+                for (int d = 0; d < D; ++d) y[d] = fma(b, K[j][d], y[d]);
+            }
+        }
+    }
+
+    // A step of size h known to be accepted (the replay of a recorded step):
+    // the operations of attempt_to's accepted branch, so (t, y) come out
+    // bit-identical, without the error estimate and the controller.  dt is
+    // left as it was.
+    __device__ __forceinline__ bool advance(double h) {
+        if (!stages(h)) return false;
+        update(h);
+        return true;
+    }
+
+    // attempt_to that also reports the step size it tried in h (dt after
+    // the clamp).
+    __device__ __forceinline__ int attempt_to(double target, bool to_stop, bool &clamped,
+                                              double &h) {
+        clamped = false;
+        const double dt_proposed = dt;
+        if (t + dt >= target) {
+            dt = target - t;
+            clamped = true;
+        }
+        h = dt;
+        if (!stages(dt)) return BG_USER_ERROR;
         // rk.rs:386-390
         double E[D];
         constexpr int e0 = Tab::e_first();
@@ -399,18 +437,7 @@ struct RkLane {
         const double tang = fma(r - 1.0, 0.125, 1.0);
         const double f = fmin(f0, tang);
         const double factor = accepted ? fmax(f, 1.0) : fmin(f, 0x1.fffffffffffffp-1);
-        // rk.rs:392-398
-        if (accepted) {
-            t += dt;
-#pragma unroll
-            for (int j = 0; j < S; ++j) {
-                const double b = Tab::b(j);
-                if (b != 0.0) {
-#pragma unroll
-                    for (int d = 0; d < D; ++d) y[d] = fma(b, K[j][d], y[d]);
-                }
-            }
-        }
+        if (accepted) update(dt);
         // dt is never NaN here (factor is finite), so the clamp is a minimum
         dt = fmin(dt * factor, dt_max);
         if (accepted && clamped && to_stop) dt = fmax(dt, dt_proposed);

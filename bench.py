@@ -49,6 +49,9 @@ FLOPS_ATTEMPT_LORENZ = 2 * 3 * (15 + 5 + 4) + 6 * (8 + 3) + 8 + 6   # 224
 FLOPS_ATTEMPT_LV = 2 * 2 * (15 + 5 + 4) + 6 * (7 + 2) + 6 + 6       # 162
 # Bogacki-Shampine 2(3): nnz(A) = 5, nnz(e) = 4, nnz(b) = 3, 4 stages
 FLOPS_ATTEMPT_LORENZ_RK23 = 2 * 3 * (5 + 4 + 3) + 4 * (8 + 3) + 8 + 6  # 130
+# replay of one accepted LV step with its recorded size (backward pass): the
+# stages and the update, 2D(nnz(A) + nnz(b)) + 6(F_rhs + D)
+FLOPS_REPLAY_STEP_LV = 2 * 2 * (15 + 4) + 6 * (7 + 2)                # 130
 # reverse of one accepted LV step: stage recompute (6 RHS, 15 nnz(A) combos,
 # 6 dt scalings) + 6 VJPs + stage-adjoint combos + b-weighting + sums
 FLOPS_VJP_STEP_LV = (6 * 7 + 2 * 2 * 15 + 6 * 2) + 6 * 20 + 2 * 2 * 15 + 2 * 2 * 4 + 6 * 2 * 2
@@ -662,8 +665,8 @@ def main():
         r = res["r"]
         ca = r["counts"].cpu().numpy().astype(np.int64)
         acc_a, att_a = int(ca[:, 0].sum()), int(ca.sum())
-        # forward attempts + backward recompute attempts + reverse steps
-        flops_a = 2 * att_a * FLOPS_ATTEMPT_LV + acc_a * FLOPS_VJP_STEP_LV
+        # forward attempts + backward replayed accepted steps + reverse steps
+        flops_a = att_a * FLOPS_ATTEMPT_LV + acc_a * (FLOPS_REPLAY_STEP_LV + FLOPS_VJP_STEP_LV)
         result["adjoint_grads_per_s"] = na / wall_a
         result["adjoint"] = {
             "workload": "discrete adjoint of batched RK45, 262,144 Lotka-Volterra instances, "

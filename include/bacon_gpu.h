@@ -370,10 +370,11 @@ int bg_adams_solve_device(int method, int rhs, int semantics, const bg_ensemble_
  * shared by all instances (increasing, in (t0, t_end]; observation times
  * before t_end become stops: the step clamp of rk.rs:366-368 aims at the
  * next one and the state lands exactly on it).  The forward solve records a
- * checkpoint (t, dt, next stop, y) every `checkpoint_every` accepted steps;
- * the backward pass re-runs each segment from its checkpoint (bit-identical
- * to the forward) and applies the reverse stage-by-stage VJP of the frozen
- * step map.  Gradients are with respect to the per-instance parameters and
+ * checkpoint (t, dt, next stop, y) every `checkpoint_every` accepted steps
+ * and the size of every accepted step; the backward pass re-runs each
+ * segment's accepted steps from its checkpoint with those sizes
+ * (bit-identical to the forward, without its rejected attempts) and applies
+ * the reverse stage-by-stage VJP of the frozen step map.  Gradients are with respect to the per-instance parameters and
  * y0.  No counterpart in the reference (it has no sensitivity API). */
 typedef struct {
     bg_ensemble_desc base;      /* inputs; status, counts, t_out, y_out (final) */
