@@ -281,19 +281,33 @@ __device__ __forceinline__ void r8_init() {
     __syncthreads();
 }
 
-__device__ __forceinline__ double root8_poly(double r, int &e) {
+// The polynomial's coefficients, highest degree first.
+constexpr double kRoot8Poly[6] = {0.0029682975037312034, -0.005693672682686368,
+                                  0.010621966427993313,  -0.025512534910039072,
+                                  0.08766623673998887,   1.0519887150881322};
+
+// c: kRoot8Poly, each through an SGPR pair (kc) for the three-address fma.
+__device__ __forceinline__ double root8_poly(double r, int &e, const double (&c)[6]) {
     int ex;
     const double mant = frexp(r, &ex);
     // mant * 2 is exact, so the fused form rounds like mul-then-add
     const double x = fma(mant, 2.0, -1.5);
     e = ex - 1;
+    double p = c[0];
+#pragma unroll
+    for (int k = 1; k < 6; ++k) p = fma(p, x, c[k]);
+    return p;
+}
+
+__device__ __forceinline__ double root8_poly(double r, int &e) {
+    int ex;
+    const double mant = frexp(r, &ex);
+    const double x = fma(mant, 2.0, -1.5);
+    e = ex - 1;
     // coefficients through SGPRs: three-address fma, no accumulator copies
-    double p = kc(0.0029682975037312034);
-    p = fma(p, x, kc(-0.005693672682686368));
-    p = fma(p, x, kc(0.010621966427993313));
-    p = fma(p, x, kc(-0.025512534910039072));
-    p = fma(p, x, kc(0.08766623673998887));
-    p = fma(p, x, kc(1.0519887150881322));
+    double p = kc(kRoot8Poly[0]);
+#pragma unroll
+    for (int k = 1; k < 6; ++k) p = fma(p, x, kc(kRoot8Poly[k]));
     return p;
 }
 
@@ -304,6 +318,12 @@ __device__ __forceinline__ double root8(double r) {
 }
 
 // The same value with the scale read from r8_table.
+__device__ __forceinline__ double root8_lds(double r, const double (&c)[6]) {
+    int e;
+    const double p = root8_poly(r, e, c);
+    return ldexp(p * r8_table[e & 7], e >> 3);
+}
+
 __device__ __forceinline__ double root8_lds(double r) {
     int e;
     const double p = root8_poly(r, e);
@@ -315,13 +335,41 @@ __device__ __forceinline__ double root8_lds(double r) {
 // ---------------------------------------------------------------------------
 enum : int { STEP_ACCEPTED = -1, STEP_REJECTED = -2 };
 
-template <class RHS, class Tab>
+// HOIST: the tableau and root coefficients are put through SGPR pairs once,
+// where the lane state is declared, rather than at each use (see cm below);
+// for the persistent ensemble loop, whose SGPR budget has room for them.
+template <class RHS, class Tab, bool HOIST = false>
 struct RkLane {
     static constexpr int D = RHS::D, S = Tab::S;
     double t, dt, end, tol, dt_min, dt_max;
     double y[D];
     double K[S][D];  // half_steps; persists across attempts (read by AS_BUILT)
     double p[RHS::P > 0 ? RHS::P : 1];
+    // With HOIST, the stage coefficients, the first error weight and the
+    // root's polynomial through SGPR pairs (kc), taken once where the lane
+    // state is declared, so every attempt reads the pairs directly instead
+    // of copying each constant into a fresh pair at its use (25 scalar moves
+    // per Lorenz RK45 attempt).  Without, kc at each use, which keeps fewer
+    // SGPRs live in kernels that have other uses for them.
+    double cm[S][S];
+    double ce0;
+    double cr[6];
+
+    __device__ RkLane() {
+        if constexpr (HOIST) {
+#pragma unroll
+            for (int i = 0; i < S; ++i)
+#pragma unroll
+                for (int j = 0; j < S; ++j) cm[i][j] = kc(Tab::m(i, j));
+            ce0 = kc(Tab::e(Tab::e_first()));
+#pragma unroll
+            for (int k = 0; k < 6; ++k) cr[k] = kc(kRoot8Poly[k]);
+        }
+    }
+    __device__ __forceinline__ double coef_m(int i, int j) const {
+        if constexpr (HOIST) return cm[i][j];
+        return kc(Tab::m(i, j));
+    }
 
     // rk.rs:361-423 without the Done test (the caller does it).  Returns
     // STEP_ACCEPTED, STEP_REJECTED or an IVPError code.
@@ -355,7 +403,7 @@ struct RkLane {
                 const double m = Tab::m(i, j);
                 if (m != 0.0) {
 #pragma unroll
-                    for (int d = 0; d < D; ++d) Y[d] = fma(kc(m), K[j][d], Y[d]);
+                    for (int d = 0; d < D; ++d) Y[d] = fma(coef_m(i, j), K[j][d], Y[d]);
                 }
             }
             const double ti = fma(Tab::c(i), h, t);
@@ -406,7 +454,7 @@ struct RkLane {
         double E[D];
         constexpr int e0 = Tab::e_first();
 #pragma unroll
-        for (int d = 0; d < D; ++d) E[d] = kc(Tab::e(e0)) * K[e0][d];
+        for (int d = 0; d < D; ++d) E[d] = (HOIST ? ce0 : kc(Tab::e(e0))) * K[e0][d];
 #pragma unroll
         for (int j = e0 + 1; j < S; ++j) {
             const double e = Tab::e(j);
@@ -426,7 +474,12 @@ struct RkLane {
         const double q2 = q * q;
         const bool accepted = ss <= q2;
         const double r = q2 / ss;
-        const double f0 = r <= 1e-8 ? 0.1 : (r >= 65536.0 ? 4.0 : root8_lds(r));
+        double f0;
+        if constexpr (HOIST) {
+            f0 = r <= 1e-8 ? 0.1 : (r >= 65536.0 ? 4.0 : root8_lds(r, cr));
+        } else {
+            f0 = r <= 1e-8 ? 0.1 : (r >= 65536.0 ? 4.0 : root8_lds(r));
+        }
         // near r = 1 the sign of delta - 1 ~ (r - 1)/8 is far below root8's
         // error: bound by the tangent 1 + (r - 1)/8 (r^(1/8) is concave, the
         // tangent is exact to second order at 1), then keep the exact power's

@@ -28,8 +28,8 @@ struct Tag {
     using type = T;
 };
 
-template <class RHS, class Tab>
-__device__ __forceinline__ void rk_load(RkLane<RHS, Tab> &L, const bg_ensemble_desc &d,
+template <class RHS, class Tab, bool H>
+__device__ __forceinline__ void rk_load(RkLane<RHS, Tab, H> &L, const bg_ensemble_desc &d,
                                         uint64_t i) {
     constexpr int D = RHS::D, S = Tab::S;
     const uint64_t n = d.n;
@@ -49,8 +49,8 @@ __device__ __forceinline__ void rk_load(RkLane<RHS, Tab> &L, const bg_ensemble_d
         for (int k = 0; k < D; ++k) L.K[s][k] = 0.0;  // rk.rs:323-327
 }
 
-template <class RHS, class Tab>
-__device__ __forceinline__ void rk_store(const RkLane<RHS, Tab> &L, const bg_ensemble_desc &d,
+template <class RHS, class Tab, bool H>
+__device__ __forceinline__ void rk_store(const RkLane<RHS, Tab, H> &L, const bg_ensemble_desc &d,
                                          uint64_t i, int status, uint32_t nacc, uint32_t nrej) {
     constexpr int D = RHS::D;
     const uint64_t n = d.n;
@@ -70,8 +70,8 @@ __device__ __forceinline__ void rk_store(const RkLane<RHS, Tab> &L, const bg_ens
 
 // One attempt plus the iterator's bookkeeping.  Returns -1 to continue, or
 // the final status (BG_OK at Done).
-template <class RHS, class Tab>
-__device__ __forceinline__ int rk_advance(RkLane<RHS, Tab> &L, uint32_t &attempts, uint32_t cap,
+template <class RHS, class Tab, bool H>
+__device__ __forceinline__ int rk_advance(RkLane<RHS, Tab, H> &L, uint32_t &attempts, uint32_t cap,
                                           uint32_t &nacc, uint32_t &nrej) {
     if (attempts >= cap) return BG_MAXIMUM_ITERATIONS_EXCEEDED;
     const int st = L.attempt();
@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(256)
     rk_ensemble_kernel(const bg_ensemble_desc *dv, uint32_t *__restrict__ counter,
                        uint32_t refill_min, uint32_t prio_cutoff) {
     r8_init();
-    RkLane<RHS, Tab> L;
+    RkLane<RHS, Tab, true> L;
     const int lane = __lane_id();
     const uint64_t max_att = dv->max_attempts;
     const uint32_t cap = (uint32_t)((max_att && max_att < kDefaultMaxAttempts) ? max_att
