@@ -331,6 +331,23 @@ __device__ __forceinline__ double root8_lds(double r) {
 }
 
 // ---------------------------------------------------------------------------
+// IEEE minNum / maxNum on values that are never signalling NaNs (results of
+// arithmetic): v_min_f64 / v_max_f64 directly.  fmin/fmax give the same
+// values but first quiet each operand the compiler cannot prove quiet (a
+// v_max_f64 x, x per operand).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double qmin(double a, double b) {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ double qmax_one(double a) {
+    double r;
+    asm("v_max_f64 %0, %1, 1.0" : "=v"(r) : "v"(a));
+    return r;
+}
+
+// ---------------------------------------------------------------------------
 // One lane's RK state and one call of RungeKuttaSolver::step.
 // ---------------------------------------------------------------------------
 enum : int { STEP_ACCEPTED = -1, STEP_REJECTED = -2 };
@@ -354,6 +371,7 @@ struct RkLane {
     double cm[S][S];
     double ce0;
     double cr[6];
+    double c_below_one;  // 1 - 2^-53
 
     __device__ RkLane() {
         if constexpr (HOIST) {
@@ -362,6 +380,7 @@ struct RkLane {
 #pragma unroll
                 for (int j = 0; j < S; ++j) cm[i][j] = kc(Tab::m(i, j));
             ce0 = kc(Tab::e(Tab::e_first()));
+            c_below_one = 0x1.fffffffffffffp-1;
 #pragma unroll
             for (int k = 0; k < 6; ++k) cr[k] = kc(kRoot8Poly[k]);
         }
@@ -488,11 +507,22 @@ struct RkLane {
         // gets the factor 1 - 2^-53, so dt stays finite and the instance runs
         // into the attempt cap (the reference retries forever).
         const double tang = fma(r - 1.0, 0.125, 1.0);
-        const double f = fmin(f0, tang);
-        const double factor = accepted ? fmax(f, 1.0) : fmin(f, 0x1.fffffffffffffp-1);
+        double f, factor;
+        if constexpr (HOIST) {
+            f = qmin(f0, tang);
+            const double fa = qmax_one(f), fr = qmin(f, c_below_one);
+            factor = accepted ? fa : fr;
+        } else {
+            f = fmin(f0, tang);
+            factor = accepted ? fmax(f, 1.0) : fmin(f, 0x1.fffffffffffffp-1);
+        }
         if (accepted) update(dt);
         // dt is never NaN here (factor is finite), so the clamp is a minimum
-        dt = fmin(dt * factor, dt_max);
+        if constexpr (HOIST) {
+            dt = qmin(dt * factor, dt_max);
+        } else {
+            dt = fmin(dt * factor, dt_max);
+        }
         if (accepted && clamped && to_stop) dt = fmax(dt, dt_proposed);
         // rk.rs:414-416
         if (dt < dt_min && t < end) return BG_MINIMUM_TIME_DELTA_EXCEEDED;
