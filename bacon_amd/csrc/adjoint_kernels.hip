@@ -75,11 +75,19 @@ __device__ __forceinline__ const double *stage_obs(double *lds, const double *ob
     }
 }
 
-// Number of observation times strictly before `end` (they become stops).
+// Number of observation times strictly before `end` (they become stops):
+// a binary search over the increasing times, ~log2(n_obs) dependent reads
+// at each instance start instead of one per time before the end.
 __device__ __forceinline__ uint32_t count_stops(const double *obs, uint32_t n_obs, double end) {
-    uint32_t k = 0;
-    while (k < n_obs && obs[k] < end) ++k;
-    return k;
+    uint32_t lo = 0, hi = n_obs;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (obs[mid] < end)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
 }
 
 template <class RHS, class Tab, bool OBS_LDS>
