@@ -101,10 +101,16 @@ class AdjointDesc(C.Structure):
 class AdjointTuning(C.Structure):
     _fields_ = [("checkpoint_every", C.c_uint32), ("max_checkpoints", C.c_uint32),
                 ("blocks_per_cu", C.c_uint32), ("n_trials", C.c_uint32),
-                ("default_ms", C.c_double), ("best_ms", C.c_double)]
+                ("default_ms", C.c_double), ("best_ms", C.c_double),
+                ("schedule", C.c_uint32), ("pad", C.c_uint32), ("forward", Tuning)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("pad", "forward")}
+        if self.schedule:
+            f = self.forward
+            d["forward_model"] = {"coef": [float(c) for c in f.coef[:3 + f.n_params]],
+                                  "pilot_n": f.pilot_n}
+        return d
 
 
 # (name, restype, argtypes) of every exported entry point
@@ -182,6 +188,8 @@ SIGNATURES = [
      [C.c_int, C.c_int, C.c_int, C.POINTER(EnsembleDesc), C.c_void_p]),
     ("bg_rk_vjp_device", C.c_int,
      [C.c_int, C.c_int, C.POINTER(AdjointDesc), C.c_void_p]),
+    ("bg_adjoint_solve_tuned_device", C.c_int,
+     [C.c_int, C.c_int, C.POINTER(AdjointDesc), C.POINTER(AdjointTuning), C.c_void_p]),
     ("bg_adjoint_autotune_device", C.c_int,
      [C.c_int, C.c_int, C.POINTER(AdjointDesc), C.POINTER(AdjointTuning), C.c_void_p]),
     ("bg_version", C.c_int, []),

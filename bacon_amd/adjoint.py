@@ -40,10 +40,12 @@ class Schedule:
 
 
 def _vjp_call(rhs, y0, params, obs_t, targets, t_end, tol, dt_min, dt_max, t0,
-              checkpoint_every, max_checkpoints, stream, order, blocks_per_cu, tuning=None):
+              checkpoint_every, max_checkpoints, stream, order, blocks_per_cu, tuning=None,
+              plan=None):
     """Builds the adjoint descriptor and output tensors, then runs
-    bg_rk_vjp_device, or bg_adjoint_autotune_device when `tuning` (an
-    L.AdjointTuning) is given."""
+    bg_rk_vjp_device, bg_adjoint_autotune_device when `tuning` (an
+    L.AdjointTuning to fill) is given, or bg_adjoint_solve_tuned_device with
+    `plan` (a filled one)."""
     dev = y0.device
     D, n = y0.shape
     P = params.shape[0]
@@ -73,7 +75,11 @@ def _vjp_call(rhs, y0, params, obs_t, targets, t_end, tol, dt_min, dt_max, t0,
                       obs_y=_ptr(out["obs_y"]))
     sh = C.c_void_p(stream.cuda_stream if stream is not None
                     else torch.cuda.current_stream(dev).cuda_stream)
-    if tuning is None:
+    if plan is not None:
+        rc = L.lib().bg_adjoint_solve_tuned_device(L.RK45, L.RHS[rhs], C.byref(d),
+                                                   C.byref(plan), sh)
+        name = "bg_adjoint_solve_tuned_device"
+    elif tuning is None:
         rc = L.lib().bg_rk_vjp_device(L.RK45, L.RHS[rhs], C.byref(d), sh)
         name = "bg_rk_vjp_device"
     else:
@@ -96,14 +102,13 @@ def solve_vjp(rhs: str, y0: torch.Tensor, params: torch.Tensor, obs_t: torch.Ten
     bg_rk_schedule_device with a plan tuned on the same problem.  With a
     `Schedule` instead, the solve uses the order it carries (if any) and
     leaves it the order for the next solve.  `plan` (from `autotune_vjp`)
-    replaces checkpoint_every, max_checkpoints and blocks_per_cu."""
+    replaces checkpoint_every, max_checkpoints and blocks_per_cu and, without
+    an `order`, may order the forward pass by its fitted cost model
+    (bg_adjoint_solve_tuned_device)."""
     if schedule is not None and order is None:
         order = schedule.current(y0.shape[1])
-    if plan is not None:
-        checkpoint_every, max_checkpoints = plan.checkpoint_every, plan.max_checkpoints
-        blocks_per_cu = plan.blocks_per_cu
     out = _vjp_call(rhs, y0, params, obs_t, targets, t_end, tol, dt_min, dt_max, t0,
-                    checkpoint_every, max_checkpoints, stream, order, blocks_per_cu)
+                    checkpoint_every, max_checkpoints, stream, order, blocks_per_cu, plan=plan)
     if schedule is not None:
         schedule.update(out["counts"], stream)
     return out
@@ -114,7 +119,8 @@ def autotune_vjp(rhs: str, y0: torch.Tensor, params: torch.Tensor, obs_t: torch.
                  t0: float = 0.0, checkpoint_every: int = 16, max_checkpoints: int = 128,
                  stream=None, order=None, blocks_per_cu: int = 0) -> "L.AdjointTuning":
     """The adjoint's plan (bg_adjoint_autotune_device): the fastest checkpoint
-    interval and resident blocks per CU for this problem, timed on the whole
+    interval, resident blocks per CU and (without an `order`) forward order,
+    input or fitted longest-first, for this problem, timed on the whole
     gradient solve; the arguments are solve_vjp's.  Returns L.AdjointTuning
     (pass it to solve_vjp as `plan`)."""
     tn = L.AdjointTuning()

@@ -874,6 +874,12 @@ extern "C" int bg_rk_vjp_device(int method, int rhs, const bg_adjoint_desc *d, v
     return rk_launch_adjoint(method, rhs, *d, (hipStream_t)stream);
 }
 
+extern "C" int bg_adjoint_solve_tuned_device(int method, int rhs, const bg_adjoint_desc *d,
+                                             const bg_adjoint_tuning *plan, void *stream) {
+    if (!d || !plan || method != BG_RK45) return BG_ERR_INVALID_ARGUMENT;
+    return adjoint_solve_tuned(method, rhs, *d, *plan, (hipStream_t)stream);
+}
+
 extern "C" int bg_adjoint_autotune_device(int method, int rhs, const bg_adjoint_desc *d,
                                           bg_adjoint_tuning *out, void *stream) {
     if (!d || !out || method != BG_RK45) return BG_ERR_INVALID_ARGUMENT;

@@ -123,6 +123,8 @@ int bdf_sys_launch_path(int method, int rhs, int semantics, uint32_t dim, BdfPat
 
 // adjoint (adjoint_kernels.hip), and its tuner (tune.hip)
 int rk_launch_adjoint(int method, int rhs, const bg_adjoint_desc &d, hipStream_t stream);
+int adjoint_solve_tuned(int method, int rhs, const bg_adjoint_desc &d,
+                        const bg_adjoint_tuning &plan, hipStream_t stream);
 int adjoint_autotune(int method, int rhs, const bg_adjoint_desc &d, bg_adjoint_tuning &out,
                      hipStream_t stream);
 

@@ -471,9 +471,39 @@ int bdf_solve_tuned(int method, int rhs, int semantics, int lu_fp32, const bg_en
     return rc;
 }
 
+// The plan's gradient solve: its K, capacity and blocks per CU, and the
+// fitted forward order when it has one and the caller gave none.
+static int adjoint_run_plan(int method, int rhs, const bg_adjoint_desc &din, uint32_t K,
+                            uint32_t maxck, uint32_t bpc, const bg_tuning *model,
+                            hipStream_t stream) {
+    bg_adjoint_desc d = din;
+    d.checkpoint_every = K;
+    d.max_checkpoints = maxck;
+    d.blocks_per_cu = bpc;
+    if (!model || d.base.order || d.base.n < 2) return rk_launch_adjoint(method, rhs, d, stream);
+    int32_t *order = nullptr;
+    BG_HIP_CHECK(hipMallocAsync((void **)&order, d.base.n * sizeof(int32_t), stream));
+    int rc = schedule(*model, d.base, order, stream);
+    if (rc == BG_OK) {
+        d.base.order = order;
+        rc = rk_launch_adjoint(method, rhs, d, stream);
+    }
+    BG_HIP_CHECK(hipFreeAsync(order, stream));
+    return rc;
+}
+
+int adjoint_solve_tuned(int method, int rhs, const bg_adjoint_desc &d,
+                        const bg_adjoint_tuning &plan, hipStream_t stream) {
+    if (plan.schedule && plan.forward.n_params > BG_TUNE_MAX_PARAMS)
+        return BG_ERR_INVALID_ARGUMENT;
+    return adjoint_run_plan(method, rhs, d, plan.checkpoint_every, plan.max_checkpoints,
+                            plan.blocks_per_cu, plan.schedule ? &plan.forward : nullptr, stream);
+}
+
 int adjoint_autotune(int method, int rhs, const bg_adjoint_desc &din, bg_adjoint_tuning &tn,
                      hipStream_t stream) {
     memset(&tn, 0, sizeof tn);
+    const uint64_t n = din.base.n;
     const uint32_t K0 = din.checkpoint_every ? din.checkpoint_every : 16u;
     const uint32_t M0 = din.max_checkpoints ? din.max_checkpoints : 128u;
     const uint64_t steps = (uint64_t)K0 * M0;  // accepted steps the caller's plan covers
@@ -488,40 +518,88 @@ int adjoint_autotune(int method, int rhs, const bg_adjoint_desc &din, bg_adjoint
             for (int q = 0; q < nt; ++q) seen |= trials[q].K == K && trials[q].bpc == bpc;
             if (!seen && nt < 12) trials[nt++] = Trial{K, bpc};
         }
-    hipEvent_t e0, e1;
-    BG_HIP_CHECK(hipEventCreate(&e0));
-    BG_HIP_CHECK(hipEventCreate(&e1));
+    // the forward order is tuned too unless the caller owns it
+    const uint32_t P = din.base.n_params < BG_TUNE_MAX_PARAMS ? din.base.n_params
+                                                              : BG_TUNE_MAX_PARAMS;
+    const bool fit = !din.base.order && n >= 2 && (P == 0 || din.base.params);
+    bg_adjoint_desc dp = din;  // the pilot: the caller's counts or a scratch buffer
+    bg_counts *scratch_counts = nullptr;
+    if (fit && !dp.base.counts) {
+        BG_HIP_CHECK(hipMallocAsync((void **)&scratch_counts, n * sizeof(bg_counts), stream));
+        dp.base.counts = scratch_counts;
+    }
+    double *rows_d = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = BG_OK;
     double best = 1e300;
-    for (int q = 0; q < nt && rc == BG_OK; ++q) {
-        bg_adjoint_desc d = din;
-        d.checkpoint_every = trials[q].K;
-        d.max_checkpoints = (uint32_t)((steps + trials[q].K - 1) / trials[q].K);
-        d.blocks_per_cu = trials[q].bpc;
-        if (hipEventRecord(e0, stream) != hipSuccess) {
+    do {
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
             rc = BG_ERR_HIP;
             break;
         }
-        rc = rk_launch_adjoint(method, rhs, d, stream);
-        if (rc != BG_OK) break;
-        float ms = 0;
-        if (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-            rc = BG_ERR_HIP;
-            break;
+        // every (K, blocks per CU) in the caller's order, then (when fitted)
+        // in the fitted order; trial 0 is the caller's own plan, whose
+        // attempt counts fit the model
+        for (int pass = 0; pass < (fit ? 2 : 1) && rc == BG_OK; ++pass) {
+            if (pass == 1) {
+                const uint32_t m = (uint32_t)(n < 16384 ? n : 16384);
+                const uint64_t stride = n / m;
+                const uint32_t W = P + 3;
+                if (hipMallocAsync((void **)&rows_d, (size_t)m * W * sizeof(double), stream) !=
+                    hipSuccess) {
+                    rc = BG_ERR_HIP;
+                    break;
+                }
+                tune_sample_kernel<<<(m + 255) / 256, 256, 0, stream>>>(dp.base, dp.base.counts,
+                                                                       stride, m, P, rows_d);
+                std::vector<double> rows((size_t)m * W);
+                if (hipGetLastError() != hipSuccess ||
+                    hipMemcpyAsync(rows.data(), rows_d, rows.size() * sizeof(double),
+                                   hipMemcpyDeviceToHost, stream) != hipSuccess ||
+                    hipStreamSynchronize(stream) != hipSuccess) {
+                    rc = BG_ERR_HIP;
+                    break;
+                }
+                tn.forward.n_params = P;
+                tn.forward.pilot_n = m;
+                fit_model(rows, m, P, tn.forward);
+            }
+            for (int q = 0; q < nt; ++q) {
+                const uint32_t K = trials[q].K;
+                const uint32_t maxck = (uint32_t)((steps + K - 1) / K);
+                if (hipEventRecord(e0, stream) != hipSuccess) {
+                    rc = BG_ERR_HIP;
+                    break;
+                }
+                rc = adjoint_run_plan(method, rhs, pass == 0 && q == 0 ? dp : din, K, maxck,
+                                      trials[q].bpc, pass == 1 ? &tn.forward : nullptr, stream);
+                if (rc != BG_OK) break;
+                float ms = 0;
+                if (hipEventRecord(e1, stream) != hipSuccess ||
+                    hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                    rc = BG_ERR_HIP;
+                    break;
+                }
+                ++tn.n_trials;
+                if (pass == 0 && q == 0) tn.default_ms = ms;  // the caller's own plan
+                if (ms < best) {
+                    best = ms;
+                    tn.checkpoint_every = K;
+                    tn.max_checkpoints = maxck;
+                    tn.blocks_per_cu = trials[q].bpc;
+                    tn.schedule = pass == 1 ? 1u : 0u;
+                }
+            }
         }
-        ++tn.n_trials;
-        if (q == 0) tn.default_ms = ms;  // the caller's own K and blocks_per_cu
-        if (ms < best) {
-            best = ms;
-            tn.checkpoint_every = d.checkpoint_every;
-            tn.max_checkpoints = d.max_checkpoints;
-            tn.blocks_per_cu = d.blocks_per_cu;
-        }
-    }
+    } while (0);
     tn.best_ms = tn.n_trials ? best : 0.0;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    if (rows_d) hipFreeAsync(rows_d, stream);
+    if (scratch_counts) hipFreeAsync(scratch_counts, stream);
+    const hipError_t e = hipStreamSynchronize(stream);
+    if (rc == BG_OK && e != hipSuccess) rc = BG_ERR_HIP;
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
     return rc;
 }
 

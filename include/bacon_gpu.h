@@ -399,9 +399,13 @@ typedef struct {
 int bg_rk_vjp_device(int method, int rhs, const bg_adjoint_desc *d, void *stream);
 
 /* A tuned plan for the adjoint (no counterpart in the reference): the
- * checkpoint interval K and the resident blocks per CU.  Gradients do not
- * depend on either (each segment is re-run from its checkpoint with the
- * forward's operations), so the plan only changes the time. */
+ * checkpoint interval K, the resident blocks per CU and, unless the caller
+ * gave the forward pass's order, whether it runs longest first by a cost
+ * model fitted to the forward's attempts (the RK tuner's model: log
+ * attempts linear in log(t_end - t0), log(tol) and the parameters).
+ * Gradients depend on none of these (each instance's operations are the
+ * same in any order, and each segment is re-run from its checkpoint with
+ * the forward's operations), so the plan only changes the time. */
 typedef struct {
     uint32_t checkpoint_every;  /* K of the fastest trial                       */
     uint32_t max_checkpoints;   /* scaled so that K x max_checkpoints covers the
@@ -409,14 +413,26 @@ typedef struct {
     uint32_t blocks_per_cu;     /* 0 = the launcher's defaults                 */
     uint32_t n_trials;
     double default_ms;          /* the caller's K and blocks_per_cu            */
-    double best_ms;
+    double best_ms;             /* the fastest trial, its schedule included    */
+    uint32_t schedule;          /* 1: the forward pass runs in the order that
+                                   bg_rk_schedule_device(&forward, &d->base, …)
+                                   gives                                       */
+    uint32_t pad;
+    bg_tuning forward;          /* the fitted model (coef, n_params, pilot_n);
+                                   its launch, pilot_ms and default_ms are 0   */
 } bg_adjoint_tuning;
 /* Times the gradient solve of `d` for K in {d's K, 16, 32, 48} and
- * blocks_per_cu in {0, 1, 2} (distinct pairs), each trial writing valid
- * results into d's outputs; returns when they are done.  Apply the plan by
- * copying its three fields into the descriptor. */
+ * blocks_per_cu in {d's, 0, 1, 2} (distinct pairs), in d's order and, when
+ * d->base.order is NULL, also in the fitted longest-first order (the
+ * schedule's cost kernel and sort inside the timed region); each trial
+ * writes valid results into d's outputs.  Returns when they are done. */
 int bg_adjoint_autotune_device(int method, int rhs, const bg_adjoint_desc *d,
                                bg_adjoint_tuning *out, void *stream);
+/* bg_rk_vjp_device with the plan applied: its K, checkpoint capacity and
+ * blocks per CU, and (plan->schedule, d->base.order NULL) the forward order
+ * computed on the device from the fitted model, in a workspace of its own. */
+int bg_adjoint_solve_tuned_device(int method, int rhs, const bg_adjoint_desc *d,
+                                  const bg_adjoint_tuning *plan, void *stream);
 
 /* Library / device information */
 int bg_version(void);

@@ -176,22 +176,39 @@ def test_tape_in_lds_and_hbm_agree(gpu, lv):
 def test_autotune_plan_bit_identical(gpu, lv):
     """bg_adjoint_autotune_device (bacon_amd.adjoint.autotune_vjp) times the
     gradient solve over K in {16, 32, 48} and 0-2 blocks per CU: 9 distinct
-    trials from the default K = 16, each writing valid results.  Its plan
-    keeps K x max_checkpoints at the caller's 16 x 128 steps, and the solve
-    with the plan returns the default solve's bits."""
+    trials from the default K = 16 in input order, and the same 9 with the
+    forward pass in the order of the cost model fitted to the first trial's
+    attempts, each writing valid results.  Its plan keeps K x
+    max_checkpoints at the caller's 16 x 128 steps, and the solve with the
+    plan (bg_adjoint_solve_tuned_device) returns the default solve's bits,
+    with the fitted order switched on or off."""
     from bacon_amd.adjoint import autotune_vjp
     base = _gpu(lv)
     tn = autotune_vjp("lotka_volterra", _t(lv["y0"]), _t(lv["params"]), _t(lv["obs"]),
                       _t(lv["targets"]), 15.0, TOL)
     print("adjoint plan:", tn.as_dict())
-    assert tn.n_trials == 9
+    assert tn.n_trials == 18
     assert tn.checkpoint_every in (16, 32, 48) and tn.blocks_per_cu in (0, 1, 2)
     assert tn.max_checkpoints == -(-16 * 128 // tn.checkpoint_every)
     assert 0.0 < tn.best_ms <= tn.default_ms
-    g = _gpu(lv, plan=tn)
-    assert np.all(g["status"] == 0)
-    for key in ("grad_params", "grad_y0", "loss", "obs_y", "counts"):
-        assert np.array_equal(g[key], base[key]), key
+    assert tn.schedule in (0, 1) and tn.forward.n_params == 4 and tn.forward.pilot_n == N
+    for sched in (tn.schedule, 1 - tn.schedule):
+        tn.schedule = sched
+        g = _gpu(lv, plan=tn)
+        assert np.all(g["status"] == 0)
+        for key in ("grad_params", "grad_y0", "loss", "obs_y", "counts"):
+            assert np.array_equal(g[key], base[key]), (sched, key)
+
+
+def test_autotune_keeps_callers_order(gpu, lv):
+    """With the caller's forward order the tuner times only K and blocks per
+    CU (9 trials) and its plan has no schedule of its own."""
+    import torch
+    from bacon_amd.adjoint import autotune_vjp
+    order = torch.arange(N - 1, -1, -1, dtype=torch.int32, device="cuda:0")
+    tn = autotune_vjp("lotka_volterra", _t(lv["y0"]), _t(lv["params"]), _t(lv["obs"]),
+                      _t(lv["targets"]), 15.0, TOL, order=order)
+    assert tn.n_trials == 9 and tn.schedule == 0
 
 
 def test_adjoint_in_measured_order_bit_identical(gpu, lv):
