@@ -333,7 +333,7 @@ __global__ void __launch_bounds__(256)
     const uint64_t maxck = ad->max_checkpoints ? ad->max_checkpoints : 128u;
     __shared__ double obs_lds[OBS_LDS ? kObsLds : 1];
     const double *obs = stage_obs<OBS_LDS>(obs_lds, ad->obs_t, n_obs);
-    RkLane<RHS, Tab> L;
+    RkLane<RHS, Tab, true> L;  // coefficients in SGPR pairs for the replay
     double lam[D], pb[P];
     int64_t inst = -1;
     int seg = -1;

@@ -354,7 +354,9 @@ enum : int { STEP_ACCEPTED = -1, STEP_REJECTED = -2 };
 
 // HOIST: the tableau and root coefficients are put through SGPR pairs once,
 // where the lane state is declared, rather than at each use (see cm below);
-// for the persistent ensemble loop, whose SGPR budget has room for them.
+// for the persistent ensemble loop and the adjoint's backward replay, whose
+// SGPR budgets have room for them (in the adjoint's forward pass they would
+// double its SGPR spills).
 template <class RHS, class Tab, bool HOIST = false>
 struct RkLane {
     static constexpr int D = RHS::D, S = Tab::S;
