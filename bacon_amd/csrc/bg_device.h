@@ -346,6 +346,12 @@ __device__ __forceinline__ double qmax_one(double a) {
     asm("v_max_f64 %0, %1, 1.0" : "=v"(r) : "v"(a));
     return r;
 }
+// b through an SGPR pair
+__device__ __forceinline__ double qmin_s(double a, double b) {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "s"(b));
+    return r;
+}
 
 // ---------------------------------------------------------------------------
 // One lane's RK state and one call of RungeKuttaSolver::step.
@@ -509,22 +515,13 @@ struct RkLane {
         // gets the factor 1 - 2^-53, so dt stays finite and the instance runs
         // into the attempt cap (the reference retries forever).
         const double tang = fma(r - 1.0, 0.125, 1.0);
-        double f, factor;
-        if constexpr (HOIST) {
-            f = qmin(f0, tang);
-            const double fa = qmax_one(f), fr = qmin(f, c_below_one);
-            factor = accepted ? fa : fr;
-        } else {
-            f = fmin(f0, tang);
-            factor = accepted ? fmax(f, 1.0) : fmin(f, 0x1.fffffffffffffp-1);
-        }
+        const double f = qmin(f0, tang);
+        const double fa = qmax_one(f);
+        const double fr = HOIST ? qmin(f, c_below_one) : qmin_s(f, 0x1.fffffffffffffp-1);
+        const double factor = accepted ? fa : fr;
         if (accepted) update(dt);
         // dt is never NaN here (factor is finite), so the clamp is a minimum
-        if constexpr (HOIST) {
-            dt = qmin(dt * factor, dt_max);
-        } else {
-            dt = fmin(dt * factor, dt_max);
-        }
+        dt = qmin(dt * factor, dt_max);
         if (accepted && clamped && to_stop) dt = fmax(dt, dt_proposed);
         // rk.rs:414-416
         if (dt < dt_min && t < end) return BG_MINIMUM_TIME_DELTA_EXCEEDED;
