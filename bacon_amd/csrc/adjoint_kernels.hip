@@ -257,6 +257,46 @@ __global__ void adj_residual_kernel(const bg_adjoint_desc *__restrict__ ad, cons
     if (ad->loss) gptr(ad->loss)[i] = loss;
 }
 
+// The same, with each block's run of the instance-major residuals
+// (kResTile instances x n_obs x D doubles, contiguous) moved between HBM and
+// LDS by coalesced loads and stores, so the per-instance loop reads and
+// writes LDS instead of one cache line per lane per value.  Identical
+// operations, so identical values.
+static constexpr uint32_t kResTile = 128;
+template <int D>
+__global__ void __launch_bounds__(kResTile)
+    adj_residual_tiled_kernel(const bg_adjoint_desc *__restrict__ ad, const AdjWs ws) {
+    extern __shared__ double tile[];
+    const uint64_t n = ad->base.n;
+    const uint32_t n_obs = ad->n_obs, per = n_obs * D;
+    const uint64_t i0 = (uint64_t)blockIdx.x * kResTile;
+    const uint32_t ni = (uint32_t)(n - i0 < kResTile ? n - i0 : kResTile);
+    const uint64_t base = i0 * per;
+    const uint32_t total = ni * per;
+    for (uint32_t e = threadIdx.x; e < total; e += kResTile) tile[e] = ws.res[base + e];
+    __syncthreads();
+    if (threadIdx.x < ni) {
+        const uint64_t i = i0 + threadIdx.x;
+        const uint32_t reached = ws.nobs[i];
+        double *const mine = tile + (uint64_t)threadIdx.x * per;
+        double loss = 0.0;
+        for (uint32_t k = 0; k < reached; ++k) {
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                const uint64_t o = ((uint64_t)k * D + c) * n + i;
+                double *r = mine + k * D + c;
+                const double y = *r, rr = y - gptr(ad->targets)[o];
+                *r = rr;
+                loss = fma(rr, rr, loss);
+                if (ad->obs_y) gptr(ad->obs_y)[o] = y;
+            }
+        }
+        if (ad->loss) gptr(ad->loss)[i] = loss;
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < total; e += kResTile) ws.res[base + e] = tile[e];
+}
+
 // Reverse of one accepted step of the explicit map
 //   Y_i = y + sum_{j<i} a_ij K_j,  K_i = dt f(Y_i),  y' = y + sum_i b_i K_i
 // given lam = dL/dy'; returns dL/dy in lam and accumulates dL/dp into pb.
@@ -615,7 +655,14 @@ int rk_launch_adjoint(int method, int rhs, const bg_adjoint_desc &ad, hipStream_
         ws.order_b = order_b;
         adj_setup_kernel<<<1, 64, 0, stream>>>(ws.counters, dd, ad);
         fwd<<<(uint32_t)grid_f, 256, 0, stream>>>(dd, ws);
-        adj_residual_kernel<D><<<(uint32_t)((n + 255) / 256), 256, 0, stream>>>(dd, ws);
+        // residuals through LDS tiles when a tile fits 64 KiB (n_obs <= 25
+        // at D = 2 … kResTile x n_obs x D doubles)
+        const size_t res_tile = (size_t)kResTile * (ad.n_obs ? ad.n_obs : 1) * D * sizeof(double);
+        if (ad.n_obs && res_tile <= 65536)
+            adj_residual_tiled_kernel<D>
+                <<<(uint32_t)((n + kResTile - 1) / kResTile), kResTile, res_tile, stream>>>(dd, ws);
+        else
+            adj_residual_kernel<D><<<(uint32_t)((n + 255) / 256), 256, 0, stream>>>(dd, ws);
         adj_order_keys_kernel<<<(uint32_t)((n + 255) / 256), 256, 0, stream>>>(ws.nck, n, keys, idx);
         hipError_t e = rocprim::radix_sort_pairs_desc(sort_tmp, sort_bytes, keys, keys_sorted,
                                                       idx, order_b, (uint32_t)n, 0, 32, stream);
