@@ -938,7 +938,12 @@ struct BdfLane {
 #pragma unroll
             for (int j = 0; j < WJ; ++j) jinv[j] = fma(lds[h * WJ + j], w, jinv[j]);
             __builtin_amdgcn_wave_barrier();
-            s = neg_jinv_times(F);
+            // s = -Jinv_new F (bdf.rs:464-465) without a second product
+            // through LDS: with Jinv_new = Jinv + w u^T and Jinv F = -adj - s
+            // (the previous s was -Jinv F_l), -Jinv_new F = adj + s - w (u . F);
+            // equal in exact arithmetic, other roundings
+            const double uf = grp.sum(u * F);
+            s = fma(-w, uf, adj + s);
             guess += s;
             const double nrm = sqrt(grp.sum(s * s));
             if (nrm <= tol) return BG_OK;
