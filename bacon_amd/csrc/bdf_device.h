@@ -547,7 +547,10 @@ struct BdfLane {
 #pragma unroll
         for (int j = 0; j < WJ; ++j) b[j] = (LuT)lds[grp.g * (W + 1) + h * WJ + j];
         __builtin_amdgcn_wave_barrier();
-        bool used = false, ok = true;
+        bool used = false;
+        // the smallest key so far: a running minimum, so no column's key
+        // stays live until the end of the elimination
+        uint32_t key_min = ~0u;
         int my_col = 0;
         LuT my_inv = (LuT)1;
         LuT *const prow = pivot_row();
@@ -557,11 +560,12 @@ struct BdfLane {
             const LuT mk = b[kk];
             const LuT other = xor32_lu(mk);
             const LuT agk = (h == hk) ? mk : other;
-            const uint32_t key = grp.max_u32(pivot_key32(agk, !used, grp.g));
-            if (!pivot_nonzero32(key)) ok = false;
             // every lane holds the same key (both halves reduce the same
-            // column), so the pivot row's lane is wave-uniform
-            const int pl = pivot_lane32(__builtin_amdgcn_readfirstlane(key));
+            // column), so the key and the pivot row's lane are wave-uniform
+            const uint32_t key =
+                __builtin_amdgcn_readfirstlane(grp.max_u32(pivot_key32(agk, !used, grp.g)));
+            key_min = __builtin_elementwise_min(key_min, key);
+            const int pl = pivot_lane32(key);
             const bool piv = (pl == grp.g);
             if (piv) {
                 used = true;
@@ -583,6 +587,7 @@ struct BdfLane {
             }
             __builtin_amdgcn_wave_barrier();
         }
+        const bool ok = pivot_nonzero32(key_min);
 #pragma unroll
         for (int j = 0; j < WJ; ++j) b[j] *= my_inv;
 #pragma unroll
