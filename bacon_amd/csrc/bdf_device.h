@@ -570,21 +570,26 @@ struct BdfLane {
             if (piv) {
                 used = true;
                 my_col = k;
-                my_inv = pivot_recip(agk);
 #pragma unroll
                 for (int j = 0; j < WJ; ++j) prow[h * WJ + j] = b[j];
-                // column k of the inverse: pivot * (1 / pivot)
-                if (h == hk) b[kk] = (LuT)1;
+                my_inv = pivot_recip(agk);
             }
             __builtin_amdgcn_wave_barrier();
-            if (!piv) {
-                // 1 / pivot straight from the pivot lane (half 0), not via LDS
-                const LuT f = agk * readlane_lu(my_inv, pl);
+            // 1 / pivot straight from the pivot lane (half 0), not via LDS
+            const LuT inv = readlane_lu(my_inv, pl);
+            // Every lane updates, so the next column's pivot search can
+            // overlap this step's FMAs.  The pivot row's lanes read back
+            // their own row and take factor +0: fma(+0, b, b) = b for every
+            // finite b, zeros' signs included (an infinite entry turns into
+            // NaN instead of staying infinite: the inverse is not finite
+            // either way).
+            const LuT f = agk * inv;
+            const LuT nf = piv ? (LuT)0 : -f;
 #pragma unroll
-                for (int j = 0; j < WJ; ++j) b[j] = fma(-f, prow[h * WJ + j], b[j]);
-                // column k of the inverse: -a(g, k) / pivot
-                if (h == hk) b[kk] = -f;
-            }
+            for (int j = 0; j < WJ; ++j) b[j] = fma(nf, prow[h * WJ + j], b[j]);
+            // column k of the inverse: pivot * (1 / pivot) on the pivot row,
+            // -a(g, k) / pivot elsewhere
+            if (h == hk) b[kk] = piv ? (LuT)1 : -f;
             __builtin_amdgcn_wave_barrier();
         }
         const bool ok = pivot_nonzero32(key_min);
