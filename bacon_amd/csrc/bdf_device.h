@@ -558,12 +558,13 @@ struct BdfLane {
         for (int k = 0; k < W; ++k) {
             const int hk = k / WJ, kk = k % WJ;
             const LuT mk = b[kk];
+            // half hk holds column k in its own lanes: its reduction is the
+            // key (the other half reduces another column, unused), and the
+            // lane swap for the other half's copy stays off the search
+            const uint32_t key = __builtin_amdgcn_readlane(
+                grp.max_u32(pivot_key32(mk, !used, grp.g)), hk * W);
             const LuT other = xor32_lu(mk);
             const LuT agk = (h == hk) ? mk : other;
-            // every lane holds the same key (both halves reduce the same
-            // column), so the key and the pivot row's lane are wave-uniform
-            const uint32_t key =
-                __builtin_amdgcn_readfirstlane(grp.max_u32(pivot_key32(agk, !used, grp.g)));
             key_min = __builtin_elementwise_min(key_min, key);
             const int pl = pivot_lane32(key);
             const bool piv = (pl == grp.g);
