@@ -166,6 +166,15 @@ struct Grp {
     }
 };
 
+// maximum of unsigned keys over each 16-lane row (one DPP max per stage)
+__device__ __forceinline__ uint32_t row_max_u32(uint32_t v) {
+    v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, true));
+    v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, true));
+    v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, true));
+    v = __builtin_elementwise_max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, true));
+    return v;
+}
+
 // Pivot choice key: |a| with its 5 lowest mantissa bits replaced by 31 - g,
 // so the maximum key names the row with the largest |a| and the lowest lane
 // among equal ones (ties below 2^-47 relative resolve towards lower lanes).
@@ -561,8 +570,11 @@ struct BdfLane {
             // half hk holds column k in its own lanes: its reduction is the
             // key (the other half reduces another column, unused), and the
             // lane swap for the other half's copy stays off the search
-            const uint32_t key = __builtin_amdgcn_readlane(
-                grp.max_u32(pivot_key32(mk, !used, grp.g)), hk * W);
+            // (the maxima of its two 16-lane rows, combined in scalars)
+            const uint32_t rm = row_max_u32(pivot_key32(mk, !used, grp.g));
+            const uint32_t key = __builtin_elementwise_max(
+                (uint32_t)__builtin_amdgcn_readlane((int)rm, hk * W),
+                (uint32_t)__builtin_amdgcn_readlane((int)rm, hk * W + 16));
             const LuT other = xor32_lu(mk);
             const LuT agk = (h == hk) ? mk : other;
             key_min = __builtin_elementwise_min(key_min, key);
