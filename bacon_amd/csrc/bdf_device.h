@@ -530,6 +530,21 @@ struct BdfLane {
         return -acc;
     }
 
+    // All-reduce sum of the component values.  SPLIT = 2: both halves hold
+    // the same values, so half 0's two 16-lane row sums, added once,
+    // replace the cross-row swap (the same bits: r0 + r1 = r1 + r0).
+    __device__ __forceinline__ double comp_sum(double v) const {
+        if constexpr (SPLIT == 2) {
+            v += dpp_f64<0xB1>(v);
+            v += dpp_f64<0x4E>(v);
+            v += dpp_f64<0x141>(v);
+            v += dpp_f64<0x140>(v);
+            return readlane_lu(v, 0) + readlane_lu(v, 16);
+        } else {
+            return grp.sum(v);
+        }
+    }
+
     // In-place Gauss-Jordan inversion (exchange steps) of the group matrix
     // whose row g is staged in row g of the LDS buffer, with partial pivoting
     // on rows that are not swapped: column k pivots on the unused row p with
@@ -940,7 +955,7 @@ struct BdfLane {
             ++n_newton;
             const double adj = neg_jinv_times(F - Fl);
             // p = (-s)^T adj ; u = s^T Jinv (column sums through LDS)
-            const double pp = grp.sum(adj * (-s));
+            const double pp = comp_sum(adj * (-s));
 #pragma unroll
             for (int j = 0; j < WJ; ++j) lds[grp.g * (W + 1) + h * WJ + j] = jinv[j] * s;
             __builtin_amdgcn_wave_barrier();
@@ -965,10 +980,10 @@ struct BdfLane {
             // through LDS: with Jinv_new = Jinv + w u^T and Jinv F = -adj - s
             // (the previous s was -Jinv F_l), -Jinv_new F = adj + s - w (u . F);
             // equal in exact arithmetic, other roundings
-            const double uf = grp.sum(u * F);
+            const double uf = comp_sum(u * F);
             s = fma(-w, uf, adj + s);
             guess += s;
-            const double nrm = sqrt(grp.sum(s * s));
+            const double nrm = sqrt(comp_sum(s * s));
             if (nrm <= tol) return BG_OK;
         }
         return BG_MAXIMUM_ITERATIONS_EXCEEDED;
